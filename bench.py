@@ -1,0 +1,153 @@
+"""Mesh solves/s of BEST's boundary-element Stokes solve on one MI355X.
+
+One step = one whole mesh solve of the 5,120-triangle bumpy ellipsoid (semi-axes 30/20/15 A, an
+l <= 6 radial bump field of sd 0.03, level-4 icosphere; 15,360 unknowns) with the vertices already
+resident in HBM: centroid shift and areas, incenters and self blocks, the 26.2 M pair blocks with
+the deflation, LU (FP32 on MFMA, FP64 iterative refinement), the 11 right-hand sides of -v, and
+the DIFCO/VISCO reductions to D_t, D_r, the coupling tensor and CHI*V ([eta]).
+
+Prints one JSON line.  Besides the contract fields it carries the FP64-LU path's rate on the same
+mesh, the errors of both paths against the oracle (the reference's dense FP64 path restated in C
+with LAPACK), the roofline of the dominant kernel class (HIP events around every launch of the
+class during the timed steps) and the oracle's CPU rate on this host.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+METRIC = "mesh solves/s: FP32-MFMA LU+FP64 IR vs FP64 LU at 1e-10 rel err in D,[eta]"
+PEAKS = {  # MI355X published dense peaks
+    "fp64_valu_tflops": 78.6, "fp32_mfma_tflops": 157.3, "fp64_mfma_tflops": 78.6, "hbm_gbs": 8000.0,
+}
+
+
+def _package():
+    if "best_hydro_amd" in sys.modules:
+        return sys.modules["best_hydro_amd"]
+    d = os.path.join(ROOT, "best-hydro_amd")
+    spec = importlib.util.spec_from_file_location("best_hydro_amd", os.path.join(d, "__init__.py"),
+                                                  submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["best_hydro_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(tri, cores):
+    """The oracle on this host: one whole solve of the same mesh (C assembly with OpenMP, LAPACK
+    dgetrf/dgetrs through SciPy's OpenBLAS, C DIFCO/VISCO)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.lib()
+    t0 = time.perf_counter()
+    ref = O.best_run(tri.v1, tri.v2, tri.v3, tri.truearea, tri.truevol, iarea=1, cg=1.0, ivisco=1, icov=0)
+    el = time.perf_counter() - t0
+    return ref, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline and error report")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve does not shard across "
+              "GPUs; see DESIGN.md for how it would)", file=sys.stderr)
+        sys.exit(2)
+
+    cores = _cores()
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", str(cores))
+    bh = _package()
+    from best_hydro_amd import meshes as M
+
+    tri = M.bumpy_ellipsoid_mesh()
+    n = tri.nar
+    s = bh.Solver(0)
+    s.set_mesh(tri)  # inputs resident in HBM before any timing
+
+    def run(prec, steps, profile=False):
+        s.set_profiling(profile)
+        t0 = time.perf_counter()
+        res = None
+        for _ in range(steps):
+            res = s.solve_mesh(iarea=1, cg=1.0, ivisco=1, icov=0, precision=prec)
+        el = time.perf_counter() - t0
+        prof = s.profile() if profile else None
+        s.set_profiling(False)
+        return res, el, prof, s.stats()
+
+    for _ in range(args.warmup):
+        run(32, 1)
+        run(64, 1)
+    r32, el32, prof32, st32 = run(32, args.steps, profile=True)
+    r64, el64, prof64, st64 = run(64, args.steps)
+    ms32 = 1e3 * el32 / args.steps
+    ms64 = 1e3 * el64 / args.steps
+
+    # Dominant kernel class of the mixed path, by measured time.
+    flop_classes = {"pairs": ("fp64_valu_tflops", "TFLOP/s"), "gemm32": ("fp32_mfma_tflops", "TFLOP/s"),
+                    "gemm64": ("fp64_mfma_tflops", "TFLOP/s"), "diag": ("fp32_mfma_tflops", "TFLOP/s"),
+                    "trsm": ("fp32_mfma_tflops", "TFLOP/s"), "resid": ("fp64_valu_tflops", "TFLOP/s")}
+    dom = max((k for k in prof32 if prof32[k]["launches"] > 0 and k in flop_classes), key=lambda k: prof32[k]["ms"])
+    p = prof32[dom]
+    avg_ms = p["ms"] / p["launches"]
+    flops_per_launch = p["flops"] / p["launches"]
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAKS[flop_classes[dom][0]]
+    roofline = {"bound": "mfma" if dom.startswith("gemm") else "valu", "kernel_class": dom,
+                "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": None, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": p["launches"] / args.steps,
+                "flops_per_launch": flops_per_launch,
+                "share_of_step": round(p["ms"] / args.steps / ms32, 3),
+                "classes_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in prof32.items() if v["launches"]}}
+
+    out = {
+        "metric": METRIC, "value": round(1e3 / ms32, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms32, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "bumpy ellipsoid 30/20/15 A, l<=6 radial bumps sd 0.03 (seeded), level-4 icosphere",
+                   "triangles": n, "unknowns": 3 * n, "rhs": 11, "lu": "FP32 MFMA + FP64 iterative refinement",
+                   "parallelism": "single GPU"},
+        "fp64_lu": {"value": round(1e3 / ms64, 4), "unit": "solves/s", "ms_per_step": round(ms64, 3)},
+        "ratio_mixed_over_fp64": round(ms64 / ms32, 3),
+        "phases_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("t_")},
+        "ir_iterations": st32["ir_iterations"],
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        ref, el = cpu_baseline(tri, cores)
+        def err(a, b):
+            return abs(a - b) / abs(b)
+        import numpy as np
+        errors = {}
+        for name, r in (("mixed", r32), ("fp64", r64)):
+            errors[name] = {"dtrans": err(r["dtrans"], ref["dtrans"]), "dr": err(r["dr"], ref["dr"]),
+                            "chiprime_eta": err(r["chiprime"], ref["chiprime"]),
+                            "coupling_dtr": float(np.max(np.abs(r["dtr"] - ref["dtr"])) / np.max(np.abs(ref["dtr"])))}
+        out["errors_vs_fp64_oracle"] = errors
+        out["cpu_baseline"] = {"value": round(1.0 / el, 5), "unit": "solves/s", "cores": cores, "kind": "port",
+                               "sample": f"one whole {n}-triangle solve of the same mesh by the oracle (C assembly, "
+                                         f"OpenMP {cores} threads; LAPACK dgetrf/dgetrs via SciPy OpenBLAS; C DIFCO/VISCO): "
+                                         f"{el:.2f} s"}
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,324 @@
+"""best-hydro_amd: BEST's boundary-element Stokes solve on one MI355X GPU.
+
+The compute lives in ``libbesthip.so`` (HIP kernels and the C ABI declared in
+``include/best_hip.h``), built in-tree by ``build()``.  This module is the host-side mirror of the
+reference's call surface for that path:
+
+* ``hm``: the nine ``bhm_*`` functions of ``hmatrix/best_hmatrix.h`` exactly as
+  ``best/besolve.f90:48-89`` binds them (one solver per process);
+* ``Solver``: a handle with the dense GPU path -- ``tgetgr`` (TGETGR), ``factor`` (the deflation
+  and DGETRF of best/best.f:384-468), ``besolv`` (BESOLV) and ``solve_mesh`` (best/best.f:261-498
+  without file I/O);
+* ``meshes``: BEST's input format and the triangulations of the workloads.
+
+There is no CPU fallback: if the library is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libbesthip.so")
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build(jobs: int = 8) -> str:
+    """Compiles libbesthip.so for gfx950 (hipcc), in-tree."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc"), f"-j{jobs}"], check=True)
+    return LIBPATH
+
+
+class BhResult(C.Structure):
+    _fields_ = [("dtt", C.c_double * 9), ("dtr", C.c_double * 9), ("drr", C.c_double * 9), ("urc", C.c_double * 3),
+                ("udr", C.c_double * 3), ("ktt", C.c_double * 9), ("krt", C.c_double * 9), ("krr", C.c_double * 9),
+                ("drr_eig", C.c_double * 3), ("dtt_eig", C.c_double * 3), ("dr", C.c_double), ("dtrans", C.c_double),
+                ("chiprime", C.c_double), ("chi", C.c_double), ("dvisc", C.c_double * 3), ("afac", C.c_double),
+                ("totarea", C.c_double), ("centroid", C.c_double * 3), ("diagm", C.c_double), ("ncomp", C.c_int),
+                ("nopen", C.c_int), ("ndef", C.c_int)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if isinstance(v, C.Array):
+                a = np.array(v[:], dtype=np.float64)
+                out[name] = a.reshape(3, 3, order="F") if len(a) == 9 else a
+            else:
+                out[name] = v
+        return out
+
+
+class BhStats(C.Structure):
+    _fields_ = [("t_prep", C.c_double), ("t_assemble", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
+                ("t_reduce", C.c_double), ("t_total", C.c_double), ("ir_iterations", C.c_int),
+                ("ir_correction", C.c_double), ("pivoted", C.c_int), ("npad", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def lib():
+    """The loaded libbesthip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: build it with best_hydro_amd.build() (hipcc, gfx950)")
+    L = C.CDLL(LIBPATH)
+    i, d, vp, lg = C.c_int, C.c_double, C.c_void_p, C.c_long
+    L.bhm_available.restype = i
+    L.bhm_create.argtypes = [i, _dp, _dp, _dp, _dp, d, d, i]
+    L.bhm_create.restype = i
+    L.bhm_incenters.argtypes = [_dp]
+    L.bhm_entries.argtypes = [i, i, i, i, _dp, i]
+    L.bhm_diag_mean.restype = d
+    L.bhm_build.argtypes = [d, i, d, i, _dp, _ip, _dp, i, i]
+    L.bhm_build.restype = i
+    L.bhm_solve.argtypes = [i, _dp]
+    L.bhm_solve.restype = i
+    L.bhm_log_line.argtypes = [C.c_char_p, i]
+    L.bhm_log_line.restype = i
+    L.bhg_create.argtypes = [i]
+    L.bhg_create.restype = vp
+    L.bhg_destroy.argtypes = [vp]
+    L.bhg_last_error.argtypes = [vp]
+    L.bhg_last_error.restype = C.c_char_p
+    L.bhg_log_line.argtypes = [vp, C.c_char_p, i]
+    L.bhg_log_line.restype = i
+    L.bhg_compon.argtypes = [i, _dp, _dp, _dp, _ip, C.POINTER(i), C.POINTER(i)]
+    L.bhg_compon.restype = i
+    L.bhg_set_mesh.argtypes = [vp, i, _dp, _dp, _dp, d, d, vp]
+    L.bhg_set_mesh.restype = i
+    L.bhg_solve_mesh.argtypes = [vp, i, d, i, i, i, C.POINTER(BhResult)]
+    L.bhg_solve_mesh.restype = i
+    L.bhg_get_solution.argtypes = [vp, i, _dp, lg]
+    L.bhg_get_solution.restype = i
+    L.bhg_get_stats.argtypes = [vp, C.POINTER(BhStats)]
+    L.bhg_get_stats.restype = i
+    L.bhg_set_profiling.argtypes = [vp, i]
+    L.bhg_set_profiling.restype = i
+    L.bhg_get_profile.argtypes = [vp, _dp, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    L.bhg_get_profile.restype = i
+    L.bhg_set_refinement.argtypes = [vp, d, i]
+    L.bhg_set_refinement.restype = i
+    L.bhg_load_prepared.argtypes = [vp, i, _dp, _dp, _dp, _dp, d, d]
+    L.bhg_load_prepared.restype = i
+    L.bhg_incenters.argtypes = [vp, _dp]
+    L.bhg_incenters.restype = i
+    L.bhg_entries.argtypes = [vp, i, i, i, i, _dp, lg]
+    L.bhg_entries.restype = i
+    L.bhg_diag_mean.argtypes = [vp]
+    L.bhg_diag_mean.restype = d
+    L.bhg_tgetgr.argtypes = [vp, _dp, _dp, lg]
+    L.bhg_tgetgr.restype = i
+    L.bhg_factor_deflated.argtypes = [vp, i, vp, vp, vp, i]
+    L.bhg_factor_deflated.restype = i
+    L.bhg_besolv.argtypes = [vp, i, _dp, lg]
+    L.bhg_besolv.restype = i
+    _LIB = L
+    return L
+
+
+def _col(v):
+    """NAR x 3 -> BEST's flat column-major buffer."""
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).T).ravel()
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One GPU solver handle (``bhg_*`` in include/best_hip.h)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        self.h = self.L.bhg_create(int(device))
+        if not self.h:
+            raise HipError(f"no HIP device {device}")
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.L.bhg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise HipError(f"{what}: {self.L.bhg_last_error(self.h).decode(errors='replace')}")
+
+    def log(self):
+        out, buf = [], C.create_string_buffer(512)
+        while True:
+            k = self.L.bhg_log_line(self.h, buf, 512)
+            if k < 0:
+                return out
+            out.append(buf.raw[:k].decode(errors="replace"))
+
+    # -- the whole mesh solve
+    def set_mesh(self, tri, icomp=None):
+        """A triangulation as BEST reads it (meshes.Triangulation or an object with v1, v2, v3,
+        truearea, truevol)."""
+        self.n = len(tri.v1)
+        ic = None
+        if icomp is not None:
+            self._icomp = np.ascontiguousarray(icomp, dtype=np.int32)
+            ic = self._icomp.ctypes.data_as(C.c_void_p)
+        self._check(self.L.bhg_set_mesh(self.h, self.n, _col(tri.v1), _col(tri.v2), _col(tri.v3), float(tri.truearea),
+                                        float(tri.truevol), ic), "set_mesh")
+
+    def solve_mesh(self, iarea=1, cg=1.0, ivisco=1, icov=0, precision=32, wm=None):
+        """best/best.f:261-498 on the resident mesh; returns the .be values and tensors."""
+        res = BhResult()
+        self._check(self.L.bhg_solve_mesh(self.h, int(iarea), float(cg), int(ivisco), int(icov), int(precision),
+                                          C.byref(res)), "solve_mesh")
+        out = res.as_dict()
+        if wm is not None and ivisco:
+            out["eta"] = out["chiprime"] * 0.602214 / wm
+        return out
+
+    def solution(self, nrhs=11):
+        out = np.zeros(3 * self.n * nrhs)
+        self._check(self.L.bhg_get_solution(self.h, nrhs, out, 3 * self.n), "solution")
+        return out.reshape(nrhs, 3 * self.n).T
+
+    def stats(self):
+        s = BhStats()
+        self.L.bhg_get_stats(self.h, C.byref(s))
+        return s.as_dict()
+
+    PROFILE_CLASSES = ("pairs", "gemm64", "gemm32", "diag", "trsm", "resid", "other")
+
+    def set_profiling(self, on=True):
+        self.L.bhg_set_profiling(self.h, int(bool(on)))
+
+    def profile(self):
+        ms, fl = np.zeros(7), np.zeros(7)
+        cnt = np.zeros(7, dtype=np.int64)
+        self.L.bhg_get_profile(self.h, ms, fl, cnt)
+        return {k: dict(ms=float(ms[i]), flops=float(fl[i]), launches=int(cnt[i]))
+                for i, k in enumerate(self.PROFILE_CLASSES)}
+
+    def set_refinement(self, tol, maxit=30):
+        self.L.bhg_set_refinement(self.h, float(tol), int(maxit))
+
+    # -- the dense path piece by piece (the besolve module's twins)
+    def load_prepared(self, n, v1, v2, v3, area, afac, cg=1.0):
+        """The mesh as bhm_create receives it: flat column-major shifted vertices, corrected areas."""
+        self.n = int(n)
+        self._check(self.L.bhg_load_prepared(self.h, self.n, np.ascontiguousarray(v1, dtype=np.float64),
+                                             np.ascontiguousarray(v2, dtype=np.float64),
+                                             np.ascontiguousarray(v3, dtype=np.float64),
+                                             np.ascontiguousarray(area, dtype=np.float64), float(afac), float(cg)),
+                    "load_prepared")
+
+    def incenters(self):
+        x = np.zeros(3 * self.n)
+        self._check(self.L.bhg_incenters(self.h, x), "incenters")
+        return x
+
+    def entries(self, i0, m, j0, nc):
+        a = np.zeros(m * nc)
+        self._check(self.L.bhg_entries(self.h, int(i0), int(m), int(j0), int(nc), a, int(m)), "entries")
+        return a.reshape(nc, m).T
+
+    def diag_mean(self):
+        return self.L.bhg_diag_mean(self.h)
+
+    def tgetgr(self):
+        n3 = 3 * self.n
+        x = np.zeros(3 * self.n)
+        gr = np.zeros(n3 * n3)
+        self._check(self.L.bhg_tgetgr(self.h, x, gr, n3), "tgetgr")
+        return x, gr.reshape(n3, n3).T
+
+    def factor(self, precision=32, ndef=1, gdef=None, comp=None, xmu=None):
+        keep = []
+
+        def ptr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        self._check(self.L.bhg_factor_deflated(self.h, int(ndef), ptr(gdef, np.float64), ptr(comp, np.int32),
+                                               ptr(xmu, np.float64), int(precision)), "factor")
+
+    def besolv(self, b):
+        b = np.asfortranarray(b, dtype=np.float64).copy()
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        flat = b.reshape(-1, order="F")
+        self._check(self.L.bhg_besolv(self.h, nrhs, flat, 3 * self.n), "besolv")
+        return flat.reshape(b.shape, order="F")
+
+
+class _HM:
+    """The nine functions of hmatrix/best_hmatrix.h, as best/besolve.f90 calls them."""
+
+    @staticmethod
+    def available():
+        return lib().bhm_available()
+
+    @staticmethod
+    def create(n, v1, v2, v3, area, afac, cg, threads=0):
+        return lib().bhm_create(int(n), v1, v2, v3, area, float(afac), float(cg), int(threads))
+
+    @staticmethod
+    def incenters(n):
+        x = np.zeros(3 * n)
+        lib().bhm_incenters(x)
+        return x
+
+    @staticmethod
+    def entries(i0, m, j0, nc):
+        a = np.zeros(m * nc)
+        lib().bhm_entries(int(i0), int(m), int(j0), int(nc), a, int(m))
+        return a.reshape(nc, m).T
+
+    @staticmethod
+    def diag_mean():
+        return lib().bhm_diag_mean()
+
+    @staticmethod
+    def build(eps, direct, eps_pre, ndef, gdef, comp, xmu, single, nrhs):
+        return lib().bhm_build(float(eps), int(direct), float(eps_pre), int(ndef),
+                               np.ascontiguousarray(gdef, dtype=np.float64), np.ascontiguousarray(comp, dtype=np.int32),
+                               np.ascontiguousarray(xmu, dtype=np.float64), int(single), int(nrhs))
+
+    @staticmethod
+    def solve(b):
+        b = np.asfortranarray(b, dtype=np.float64).copy()
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        flat = b.reshape(-1, order="F")
+        rc = lib().bhm_solve(nrhs, flat)
+        return rc, flat.reshape(b.shape, order="F")
+
+    @staticmethod
+    def log_lines():
+        out, buf = [], C.create_string_buffer(512)
+        while True:
+            k = lib().bhm_log_line(buf, 512)
+            if k < 0:
+                return out
+            out.append(buf.raw[:k].decode(errors="replace"))
+
+    @staticmethod
+    def destroy():
+        lib().bhm_destroy()
+
+
+hm = _HM()

@@ -1,0 +1,120 @@
+// Shared definitions of the MI355X BEST solver library (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace bhg {
+
+// LU block size: the diagonal block of the blocked LU lives in LDS (128 x 128 FP64 = 128 KiB of
+// the 160 KiB per CU) and every GEMM tile edge is a multiple of it, so matrices are padded to a
+// multiple of kNB with an identity block.
+constexpr int kNB = 128;
+
+// Per-source-triangle record used by the pair kernel: the vertices, the corrected area and the
+// source-only quantities of the quadrature (edge vectors, their dot products, sqrt(d2121)).
+// 32 doubles so that one record is two 128-byte lines and loads as scalars (uniform per wave).
+constexpr int kSrcStride = 32;
+enum SrcField {
+   kP1 = 0,    // p1 x,y,z
+   kP2 = 3,    // p2
+   kP3 = 6,    // p3
+   kArea = 9,  // corrected area
+   kV21 = 10,  // p2 - p1
+   kV31 = 13,  // p3 - p1
+   kV32 = 16,  // p3 - p2
+   kD2121 = 19,
+   kD2131 = 20,
+   kD2132 = 21,
+   kD3131 = 22,
+   kD3232 = 23,
+   kD3231 = 24,
+   kS21 = 25,  // sqrt(d2121)
+};
+
+// Kernel classes timed by the optional launch profiler.
+enum KernelClass { kKPairs = 0, kKGemm64, kKGemm32, kKDiag, kKTrsm, kKResid, kKOther, kKClasses };
+
+// When a solver turns profiling on, every launch of a timed class is bracketed by two HIP events
+// on the launching stream; the algorithmic flops of the launch are added to its class.  The
+// events come from a pool that grows once and is reused.
+struct LaunchProfiler {
+   bool on = false;
+   hipStream_t st = nullptr;
+   std::vector<hipEvent_t> pool;
+   size_t used = 0;
+   std::vector<int> cls;
+   std::vector<double> flops;
+   double total_ms[ kKClasses ] = {};
+   double total_flops[ kKClasses ] = {};
+   long count[ kKClasses ] = {};
+   void reset() {
+      used = 0;
+      cls.clear();
+      flops.clear();
+      for ( int k = 0; k < kKClasses; ++k ) total_ms[ k ] = total_flops[ k ] = 0, count[ k ] = 0;
+   }
+   hipEvent_t next() {
+      if ( used == pool.size() ) {
+         hipEvent_t e;
+         if ( hipEventCreate( &e ) != hipSuccess ) throw std::runtime_error( "hipEventCreate" );
+         pool.push_back( e );
+      }
+      return pool[ used++ ];
+   }
+   void begin( int k, double f ) {
+      if ( !on ) return;
+      (void)hipEventRecord( next(), st );
+      cls.push_back( k );
+      flops.push_back( f );
+   }
+   void end() {
+      if ( !on ) return;
+      (void)hipEventRecord( next(), st );
+   }
+   // After the stream has been synchronized.
+   void collect() {
+      for ( size_t i = 0; i < cls.size(); ++i ) {
+         float ms = 0.f;
+         if ( hipEventElapsedTime( &ms, pool[ 2 * i ], pool[ 2 * i + 1 ] ) != hipSuccess ) continue;
+         total_ms[ cls[ i ] ] += ms;
+         total_flops[ cls[ i ] ] += flops[ i ];
+         count[ cls[ i ] ] += 1;
+      }
+      used = 0;
+      cls.clear();
+      flops.clear();
+   }
+   ~LaunchProfiler() {
+      for ( auto e : pool ) (void)hipEventDestroy( e );
+   }
+};
+
+// The profiler of the solver whose work is being enqueued on this host thread (or null).
+extern thread_local LaunchProfiler* g_prof;
+
+struct ProfScope {
+   explicit ProfScope( int k, double f ) {
+      if ( g_prof ) g_prof->begin( k, f );
+   }
+   ~ProfScope() {
+      if ( g_prof ) g_prof->end();
+   }
+};
+
+inline void hip_check( hipError_t e, const char* what, const char* file, int line ) {
+   if ( e != hipSuccess ) {
+      char buf[ 512 ];
+      std::snprintf( buf, sizeof buf, "%s failed at %s:%d: %s", what, file, line, hipGetErrorString( e ) );
+      throw std::runtime_error( buf );
+   }
+}
+
+}  // namespace bhg
+
+#define BH_HIP( x ) ::bhg::hip_check( ( x ), #x, __FILE__, __LINE__ )

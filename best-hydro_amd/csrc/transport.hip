@@ -1,0 +1,441 @@
+// The right-hand sides BEST solves for and the reductions of its solutions, on the GPU; the 3x3
+// algebra that turns the reductions into BEST's tensors, on the host (FP64 throughout).
+//
+//   k_rhs           best/difco.f:28-44 (six rigid-body flows) and best/visco.f:216-243 (STRAIN,
+//                   five strain flows), as 11 columns of one right-hand-side block
+//   k_sums          best/difco.f:73-98 (forces, torques), best/visco.f:76-129 (BB, C of the centre
+//                   of viscosity) and :191-205 (the CHI moments, with the shift to the centre of
+//                   viscosity applied on the fly): per-workgroup partial sums
+//   difco_algebra   best/difco.f:106-401 with MATIN3 (best/matin3.f)
+//   visco_chiprime  best/visco.f:118-211
+//   output_values   best/output.f:16-29, :98-167 with JACOBI (best/jacobi.f)
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bh_common.h"
+#include "transport.h"
+
+namespace bhg {
+
+// Column layout of the right-hand-side block.
+constexpr int kRhsRigid = 0;   // 6 columns
+constexpr int kRhsStrain = 6;  // 5 columns
+
+__global__ void k_rhs( int n, const double* __restrict__ xc, double* __restrict__ b, long ldb, int with_strain ) {
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if ( i >= n ) return;
+   const double x1 = xc[ i ], x2 = xc[ n + i ], x3 = xc[ 2 * n + i ];
+   const long i0 = 3 * (long)i;
+   double* c[ 11 ];
+   for ( int k = 0; k < 11; ++k ) c[ k ] = b + (long)k * ldb + i0;
+   for ( int k = 0; k < 6; ++k ) c[ k ][ 0 ] = c[ k ][ 1 ] = c[ k ][ 2 ] = 0.0;
+   c[ 0 ][ 0 ] = 1.0;
+   c[ 1 ][ 1 ] = 1.0;
+   c[ 2 ][ 2 ] = 1.0;
+   c[ 3 ][ 1 ] = -x3;
+   c[ 3 ][ 2 ] = x2;
+   c[ 4 ][ 0 ] = x3;
+   c[ 4 ][ 2 ] = -x1;
+   c[ 5 ][ 0 ] = -x2;
+   c[ 5 ][ 1 ] = x1;
+   if ( !with_strain ) return;
+   double** s = c + kRhsStrain;
+   s[ 0 ][ 0 ] = 0.5 * x2, s[ 0 ][ 1 ] = 0.5 * x1, s[ 0 ][ 2 ] = 0.0;
+   s[ 1 ][ 0 ] = 0.5 * x3, s[ 1 ][ 1 ] = 0.0, s[ 1 ][ 2 ] = 0.5 * x1;
+   s[ 2 ][ 0 ] = 0.0, s[ 2 ][ 1 ] = 0.5 * x3, s[ 2 ][ 2 ] = 0.5 * x2;
+   s[ 3 ][ 0 ] = 0.5 * x1, s[ 3 ][ 1 ] = -0.5 * x2, s[ 3 ][ 2 ] = 0.0;
+   s[ 4 ][ 0 ] = 0.5 * x1, s[ 4 ][ 1 ] = 0.0, s[ 4 ][ 2 ] = -0.5 * x3;
+}
+
+// Partial sums, one set per workgroup (kSumSlots doubles each):
+//   [0, 36)   DIFCO: flow mx -> 6 mx + {0..2: -sum area F_k, 3..5: -sum area (x cross F)_k}
+//   [36, 45)  CHI112, CHI213, CHI323, CHI411, CHI422, CHI433, CHI511, CHI522, CHI533 with x and
+//             the strain solutions moved to the centre of viscosity d (d = 0: the centroid)
+//   [45, 54)  BB(1,1), BB(1,2), BB(1,3), BB(2,2), BB(2,3), BB(3,3), C1, C2, C3 (before scaling)
+__global__ void __launch_bounds__( 256 ) k_sums( int n, const double* __restrict__ xc, const double* __restrict__ area,
+                                                const double* __restrict__ b, long ldb, int nstrain, double d1, double d2,
+                                                double d3, int want_bb, double* __restrict__ partial ) {
+   __shared__ double red[ 256 ];
+   double acc[ kSumSlots ];
+   for ( int k = 0; k < kSumSlots; ++k ) acc[ k ] = 0.0;
+   const int per = ( n + gridDim.x - 1 ) / gridDim.x;
+   const int lo = blockIdx.x * per, hi = min( n, lo + per );
+   for ( int i = lo + threadIdx.x; i < hi; i += blockDim.x ) {
+      const double x1 = xc[ i ], x2 = xc[ n + i ], x3 = xc[ 2 * n + i ];
+      const double ar = area[ i ];
+      const long i0 = 3 * (long)i;
+      for ( int mx = 0; mx < 6; ++mx ) {
+         const double* f = b + (long)( kRhsRigid + mx ) * ldb + i0;
+         const double q1 = ( x2 * f[ 2 ] - x3 * f[ 1 ] ), q2 = ( x3 * f[ 0 ] - x1 * f[ 2 ] ), q3 = ( x1 * f[ 1 ] - x2 * f[ 0 ] );
+         acc[ 6 * mx + 0 ] = acc[ 6 * mx + 0 ] - ar * f[ 0 ];
+         acc[ 6 * mx + 1 ] = acc[ 6 * mx + 1 ] - ar * f[ 1 ];
+         acc[ 6 * mx + 2 ] = acc[ 6 * mx + 2 ] - ar * f[ 2 ];
+         acc[ 6 * mx + 3 ] = acc[ 6 * mx + 3 ] - ar * q1;
+         acc[ 6 * mx + 4 ] = acc[ 6 * mx + 4 ] - ar * q2;
+         acc[ 6 * mx + 5 ] = acc[ 6 * mx + 5 ] - ar * q3;
+      }
+      if ( nstrain < 5 ) continue;
+      // W(l, c) at this triangle after the move to d (best/visco.f:145-166).
+      double w[ 5 ][ 3 ];
+      const double* e1p = b + (long)( kRhsRigid + 0 ) * ldb + i0;
+      const double* e2p = b + (long)( kRhsRigid + 1 ) * ldb + i0;
+      const double* e3p = b + (long)( kRhsRigid + 2 ) * ldb + i0;
+      for ( int c = 0; c < 3; ++c ) {
+         const double e1 = e1p[ c ], e2 = e2p[ c ], e3 = e3p[ c ];
+         const double* v = b + (long)kRhsStrain * ldb + i0 + c;
+         w[ 0 ][ c ] = v[ 0 * ldb ] - 0.5 * ( d2 * e1 + d1 * e2 );
+         w[ 1 ][ c ] = v[ 1 * ldb ] - 0.5 * ( d3 * e1 + d1 * e3 );
+         w[ 2 ][ c ] = v[ 2 * ldb ] - 0.5 * ( d3 * e2 + d2 * e3 );
+         w[ 3 ][ c ] = v[ 3 * ldb ] - 0.5 * ( d1 * e1 - d2 * e2 );
+         w[ 4 ][ c ] = v[ 4 * ldb ] - 0.5 * ( d1 * e1 - d3 * e3 );
+      }
+      const double y1 = x1 - d1, y2 = x2 - d2, y3 = x3 - d3;
+      acc[ 36 ] = acc[ 36 ] + ar * ( w[ 0 ][ 0 ] * y2 + w[ 0 ][ 1 ] * y1 );
+      acc[ 37 ] = acc[ 37 ] + ar * ( w[ 1 ][ 0 ] * y3 + w[ 1 ][ 2 ] * y1 );
+      acc[ 38 ] = acc[ 38 ] + ar * ( w[ 2 ][ 1 ] * y3 + w[ 2 ][ 2 ] * y2 );
+      acc[ 39 ] = acc[ 39 ] + ar * w[ 3 ][ 0 ] * y1;
+      acc[ 40 ] = acc[ 40 ] + ar * w[ 3 ][ 1 ] * y2;
+      acc[ 41 ] = acc[ 41 ] + ar * w[ 3 ][ 2 ] * y3;
+      acc[ 42 ] = acc[ 42 ] + ar * w[ 4 ][ 0 ] * y1;
+      acc[ 43 ] = acc[ 43 ] + ar * w[ 4 ][ 1 ] * y2;
+      acc[ 44 ] = acc[ 44 ] + ar * w[ 4 ][ 2 ] * y3;
+      if ( !want_bb ) continue;
+      // G(l,k) = 2 V0(3n+k, l) and H(l,k,m) from the unit-translation solutions (visco.f:36-61).
+      double H[ 5 ][ 3 ][ 3 ], G[ 5 ][ 3 ];
+      for ( int k = 0; k < 3; ++k ) {
+         const double e1 = e1p[ k ], e2 = e2p[ k ], e3 = e3p[ k ];
+         for ( int l = 0; l < 5; ++l ) G[ l ][ k ] = 2.0 * b[ (long)( kRhsStrain + l ) * ldb + i0 + k ];
+         H[ 0 ][ k ][ 0 ] = e2, H[ 0 ][ k ][ 1 ] = e1, H[ 0 ][ k ][ 2 ] = 0.0;
+         H[ 1 ][ k ][ 0 ] = e3, H[ 1 ][ k ][ 1 ] = 0.0, H[ 1 ][ k ][ 2 ] = e1;
+         H[ 2 ][ k ][ 0 ] = 0.0, H[ 2 ][ k ][ 1 ] = e3, H[ 2 ][ k ][ 2 ] = e2;
+         H[ 3 ][ k ][ 0 ] = e1, H[ 3 ][ k ][ 1 ] = -e2, H[ 3 ][ k ][ 2 ] = 0.0;
+         H[ 4 ][ k ][ 0 ] = e1, H[ 4 ][ k ][ 1 ] = 0.0, H[ 4 ][ k ][ 2 ] = -e3;
+      }
+#define H_( l, k, m ) H[ ( l )-1 ][ ( k )-1 ][ ( m )-1 ]
+#define G_( l, k ) G[ ( l )-1 ][ ( k )-1 ]
+      acc[ 45 ] = acc[ 45 ] + ar * ( 3 * ( H_( 1, 2, 1 ) + H_( 2, 3, 1 ) ) + 2 * ( H_( 4, 1, 1 ) + H_( 5, 1, 1 ) ) );
+      acc[ 46 ] = acc[ 46 ] + ar * ( -4 * H_( 4, 2, 1 ) + 3 * ( H_( 1, 1, 1 ) + H_( 1, 2, 2 ) + H_( 2, 3, 2 ) + H_( 3, 3, 1 ) ) +
+                                     2 * ( H_( 4, 1, 2 ) + H_( 5, 1, 2 ) + H_( 5, 2, 1 ) ) );
+      acc[ 47 ] = acc[ 47 ] + ar * ( -4 * H_( 5, 3, 1 ) + 3 * ( H_( 1, 2, 3 ) + H_( 2, 1, 1 ) + H_( 2, 3, 3 ) + H_( 3, 2, 1 ) ) +
+                                     2 * ( H_( 4, 1, 3 ) + H_( 4, 3, 1 ) + H_( 5, 1, 3 ) ) );
+      acc[ 48 ] = acc[ 48 ] + ar * ( -4 * H_( 4, 2, 2 ) + 3 * ( H_( 1, 1, 2 ) + H_( 3, 3, 2 ) ) + 2 * H_( 5, 2, 2 ) );
+      acc[ 49 ] = acc[ 49 ] + ar * ( -4 * H_( 4, 2, 3 ) + 3 * ( H_( 3, 2, 2 ) + H_( 3, 3, 3 ) + H_( 1, 1, 3 ) + H_( 2, 1, 2 ) ) +
+                                     2 * ( H_( 4, 3, 2 ) + H_( 5, 2, 3 ) ) );
+      acc[ 50 ] = acc[ 50 ] + ar * ( -4 * H_( 5, 3, 3 ) + 3 * ( H_( 2, 1, 3 ) + H_( 3, 2, 3 ) ) + 2 * H_( 4, 3, 3 ) );
+      acc[ 51 ] = acc[ 51 ] + ar * ( -3 * G_( 1, 2 ) - 3 * G_( 2, 3 ) - 2 * G_( 4, 1 ) - 2 * G_( 5, 1 ) -
+                                     x1 * ( 3 * ( H_( 1, 2, 1 ) + H_( 2, 3, 1 ) ) + 2 * ( H_( 4, 1, 1 ) + H_( 5, 1, 1 ) ) ) -
+                                     x2 * ( 3 * ( H_( 1, 1, 1 ) + H_( 3, 3, 1 ) ) - 4 * H_( 4, 2, 1 ) + 2 * H_( 5, 2, 1 ) ) -
+                                     x3 * ( 3 * ( H_( 2, 1, 1 ) + H_( 3, 2, 1 ) ) - 4 * H_( 5, 3, 1 ) + 2 * H_( 4, 3, 1 ) ) );
+      acc[ 52 ] = acc[ 52 ] + ar * ( -3 * G_( 1, 1 ) - 3 * G_( 3, 3 ) + 4 * G_( 4, 2 ) - 2 * G_( 5, 2 ) -
+                                     x1 * ( 3 * ( H_( 1, 2, 2 ) + H_( 2, 3, 2 ) ) + 2 * ( H_( 4, 1, 2 ) + H_( 5, 1, 2 ) ) ) -
+                                     x2 * ( 3 * ( H_( 1, 1, 2 ) + H_( 3, 3, 2 ) ) - 4 * H_( 4, 2, 2 ) + 2 * H_( 5, 2, 2 ) ) -
+                                     x3 * ( 3 * ( H_( 2, 1, 2 ) + H_( 3, 2, 2 ) ) - 4 * H_( 5, 3, 2 ) + 2 * H_( 4, 3, 2 ) ) );
+      acc[ 53 ] = acc[ 53 ] + ar * ( -3 * G_( 2, 1 ) - 3 * G_( 3, 2 ) - 2 * G_( 4, 3 ) + 4 * G_( 5, 3 ) -
+                                     x1 * ( 3 * ( H_( 1, 2, 3 ) + H_( 2, 3, 3 ) ) + 2 * ( H_( 4, 1, 3 ) + H_( 5, 1, 3 ) ) ) -
+                                     x2 * ( 3 * ( H_( 1, 1, 3 ) + H_( 3, 3, 3 ) ) - 4 * H_( 4, 2, 3 ) + 2 * H_( 5, 2, 3 ) ) -
+                                     x3 * ( 3 * ( H_( 2, 1, 3 ) + H_( 3, 2, 3 ) ) - 4 * H_( 5, 3, 3 ) + 2 * H_( 4, 3, 3 ) ) );
+#undef H_
+#undef G_
+   }
+   // Fixed-order tree reduction over the workgroup, slot by slot.
+   for ( int k = 0; k < kSumSlots; ++k ) {
+      red[ threadIdx.x ] = acc[ k ];
+      __syncthreads();
+      for ( int w = 128; w > 0; w >>= 1 ) {
+         if ( threadIdx.x < w ) red[ threadIdx.x ] = red[ threadIdx.x ] + red[ threadIdx.x + w ];
+         __syncthreads();
+      }
+      if ( threadIdx.x == 0 ) partial[ (long)blockIdx.x * kSumSlots + k ] = red[ 0 ];
+      __syncthreads();
+   }
+}
+
+void launch_rhs( hipStream_t st, int n, const double* xc, double* b, long ldb, int with_strain ) {
+   hipLaunchKernelGGL( k_rhs, dim3( ( n + 255 ) / 256 ), dim3( 256 ), 0, st, n, xc, b, ldb, with_strain );
+}
+
+void launch_sums( hipStream_t st, int n, const double* xc, const double* area, const double* b, long ldb, int nstrain,
+                  const double d[ 3 ], int want_bb, double* partial ) {
+   hipLaunchKernelGGL( k_sums, dim3( kSumGroups ), dim3( 256 ), 0, st, n, xc, area, b, ldb, nstrain, d[ 0 ], d[ 1 ], d[ 2 ],
+                       want_bb, partial );
+}
+
+// ------------------------------------------------------------------ host 3x3 algebra
+
+namespace {
+
+// 3x3 matrices column-major: m[i + 3 j] is (i, j), 0-based.
+inline double& at( double* m, int i, int j ) { return m[ i + 3 * j ]; }
+inline double at( const double* m, int i, int j ) { return m[ i + 3 * j ]; }
+
+// The inverse of a 3x3 matrix by LU with implicit (scaled) partial pivoting and back
+// substitution per unit column (best/matin3.f).  a is overwritten.  Returns nonzero when a row is
+// all zeros.
+int inverse3( double* a, double* y ) {
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) at( y, i, j ) = i == j ? 1.0 : 0.0;
+   double scale[ 3 ];
+   for ( int i = 0; i < 3; ++i ) {
+      double big = 0.0;
+      for ( int j = 0; j < 3; ++j ) big = std::fabs( at( a, i, j ) ) > big ? std::fabs( at( a, i, j ) ) : big;
+      if ( big == 0.0 ) return 1;
+      scale[ i ] = 1.0 / big;
+   }
+   int perm[ 3 ];
+   for ( int j = 0; j < 3; ++j ) {
+      for ( int i = 1; i < j; ++i ) {
+         double s = at( a, i, j );
+         for ( int k = 0; k < i; ++k ) s = s - at( a, i, k ) * at( a, k, j );
+         at( a, i, j ) = s;
+      }
+      double big = 0.0;
+      int imax = j;
+      for ( int i = j; i < 3; ++i ) {
+         double s = at( a, i, j );
+         for ( int k = 0; k < j; ++k ) s = s - at( a, i, k ) * at( a, k, j );
+         if ( j > 0 ) at( a, i, j ) = s;
+         const double r = scale[ i ] * std::fabs( s );
+         if ( r >= big ) big = r, imax = i;
+      }
+      if ( imax != j ) {
+         for ( int k = 0; k < 3; ++k ) std::swap( at( a, imax, k ), at( a, j, k ) );
+         scale[ imax ] = scale[ j ];
+      }
+      perm[ j ] = imax;
+      if ( j != 2 ) {
+         if ( at( a, j, j ) == 0.0 ) at( a, j, j ) = 1e-20;
+         const double r = 1.0 / at( a, j, j );
+         for ( int i = j + 1; i < 3; ++i ) at( a, i, j ) = at( a, i, j ) * r;
+      }
+   }
+   if ( at( a, 2, 2 ) == 0.0 ) at( a, 2, 2 ) = 1e-20;
+   for ( int c = 0; c < 3; ++c ) {
+      double* v = y + 3 * c;
+      int first = 0;
+      for ( int i = 0; i < 3; ++i ) {
+         double s = v[ perm[ i ] ];
+         v[ perm[ i ] ] = v[ i ];
+         if ( first != 0 ) {
+            for ( int j = first - 1; j < i; ++j ) s = s - at( a, i, j ) * v[ j ];
+         } else if ( s != 0.0 ) {
+            first = 1;
+         }
+         v[ i ] = s;
+      }
+      for ( int i = 2; i >= 0; --i ) {
+         double s = v[ i ];
+         for ( int j = i + 1; j < 3; ++j ) s = s - at( a, i, j ) * v[ j ];
+         v[ i ] = s / at( a, i, i );
+      }
+   }
+   return 0;
+}
+
+// Cyclic Jacobi eigenvalues of a symmetric 3x3 matrix read from its upper triangle
+// (best/jacobi.f).  a is destroyed.
+void eigen3( double* a, double d[ 3 ] ) {
+   double b[ 3 ], z[ 3 ];
+   for ( int p = 0; p < 3; ++p ) b[ p ] = d[ p ] = at( a, p, p ), z[ p ] = 0.0;
+   for ( int sweep = 1; sweep <= 50; ++sweep ) {
+      double off = 0.0;
+      for ( int p = 0; p < 2; ++p )
+         for ( int q = p + 1; q < 3; ++q ) off = off + std::fabs( at( a, p, q ) );
+      if ( off == 0.0 ) return;
+      const double tresh = sweep < 4 ? 0.2 * off / 9.0 : 0.0;
+      for ( int p = 0; p < 2; ++p )
+         for ( int q = p + 1; q < 3; ++q ) {
+            const double apq = at( a, p, q );
+            const double g = 100.0 * std::fabs( apq );
+            if ( sweep > 4 && std::fabs( d[ p ] ) + g == std::fabs( d[ p ] ) && std::fabs( d[ q ] ) + g == std::fabs( d[ q ] ) ) {
+               at( a, p, q ) = 0.0;
+               continue;
+            }
+            if ( !( std::fabs( apq ) > tresh ) ) continue;
+            double h = d[ q ] - d[ p ], t;
+            if ( std::fabs( h ) + g == std::fabs( h ) ) {
+               t = apq / h;
+            } else {
+               const double theta = 0.5 * h / apq;
+               t = 1.0 / ( std::fabs( theta ) + std::sqrt( 1.0 + theta * theta ) );
+               if ( theta < 0.0 ) t = -t;
+            }
+            const double c = 1.0 / std::sqrt( 1 + t * t ), s = t * c, tau = s / ( 1.0 + c );
+            h = t * apq;
+            z[ p ] = z[ p ] - h, z[ q ] = z[ q ] + h;
+            d[ p ] = d[ p ] - h, d[ q ] = d[ q ] + h;
+            at( a, p, q ) = 0.0;
+            auto rot = [&]( double& x, double& y ) {
+               const double gx = x, hy = y;
+               x = gx - s * ( hy + gx * tau );
+               y = hy + s * ( gx - hy * tau );
+            };
+            for ( int j = 0; j < p; ++j ) rot( at( a, j, p ), at( a, j, q ) );
+            for ( int j = p + 1; j < q; ++j ) rot( at( a, p, j ), at( a, j, q ) );
+            for ( int j = q + 1; j < 3; ++j ) rot( at( a, p, j ), at( a, q, j ) );
+         }
+      for ( int p = 0; p < 3; ++p ) b[ p ] = b[ p ] + z[ p ], d[ p ] = b[ p ], z[ p ] = 0.0;
+   }
+}
+
+// uu = the matrix of the centre vector r of best/difco.f:180-188 (the cross-product matrix,
+// with r(1) at (3,2)).
+void cross_matrix( const double r[ 3 ], double* uu ) {
+   at( uu, 2, 0 ) = -r[ 1 ];
+   at( uu, 2, 1 ) = r[ 0 ];
+   at( uu, 0, 1 ) = -r[ 2 ];
+   at( uu, 1, 0 ) = -at( uu, 0, 1 );
+   at( uu, 1, 2 ) = -at( uu, 2, 1 );
+   at( uu, 0, 2 ) = -at( uu, 2, 0 );
+   at( uu, 0, 0 ) = at( uu, 1, 1 ) = at( uu, 2, 2 ) = 0.0;
+}
+
+}  // namespace
+
+void difco_algebra( const double* s, double cg, Transport& T ) {
+   double ktt[ 9 ], ktr[ 9 ], krt[ 9 ], krr[ 9 ], uu[ 9 ], aux[ 9 ], kinv[ 9 ], work[ 9 ];
+   // Friction tensors: column i of Ktt/Ktr from translation i, of Krt/Krr from rotation i.
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) {
+         at( ktt, j, i ) = -s[ 6 * i + j ] * ( 1.0 / 1.0 );
+         at( ktr, j, i ) = -s[ 6 * i + 3 + j ] * ( 1.0 / 1.0 );
+         at( krt, j, i ) = -s[ 6 * ( 3 + i ) + j ] * ( 1.0 / 1.0 );
+         at( krr, j, i ) = -s[ 6 * ( 3 + i ) + 3 + j ] * ( 1.0 / 1.0 );
+      }
+   std::memcpy( T.ktr, ktr, sizeof ktr );
+   // Centre of resistance (difco.f:141-174).
+   at( uu, 0, 0 ) = at( ktt, 1, 1 ) + at( ktt, 2, 2 );
+   at( uu, 1, 1 ) = at( ktt, 0, 0 ) + at( ktt, 2, 2 );
+   at( uu, 2, 2 ) = at( ktt, 1, 1 ) + at( ktt, 0, 0 );
+   at( uu, 0, 1 ) = -at( ktt, 0, 1 );
+   at( uu, 0, 2 ) = -at( ktt, 0, 2 );
+   at( uu, 2, 1 ) = -at( ktt, 1, 2 );
+   at( uu, 1, 0 ) = at( uu, 0, 1 );
+   at( uu, 1, 2 ) = at( uu, 2, 1 );
+   at( uu, 2, 0 ) = at( uu, 0, 2 );
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) at( krt, i, j ) = ( at( krt, i, j ) + at( ktr, j, i ) ) * 0.5;
+   const double sur1 = -at( krt, 2, 1 ) + at( krt, 1, 2 );
+   const double sur2 = -at( krt, 0, 2 ) + at( krt, 2, 0 );
+   const double sur3 = -at( krt, 1, 0 ) + at( krt, 0, 1 );
+   inverse3( uu, aux );
+   for ( int j = 0; j < 3; ++j ) T.urc[ j ] = ( at( aux, j, 0 ) * sur1 + at( aux, j, 1 ) * sur2 + at( aux, j, 2 ) * sur3 );
+   cross_matrix( T.urc, uu );
+   // Krr' = Krr - uu Ktt uu + Krt^T uu - uu Krt,  Krt' = Krt + Ktt uu  (difco.f:196-214).
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) {
+         double t = 0.0;
+         for ( int k = 0; k < 3; ++k )
+            t = t + at( uu, i, k ) * ( at( ktt, k, 0 ) * at( uu, 0, j ) + at( ktt, k, 1 ) * at( uu, 1, j ) + at( ktt, k, 2 ) * at( uu, 2, j ) );
+         const double t1 = at( krt, 0, i ) * at( uu, 0, j ) + at( krt, 1, i ) * at( uu, 1, j ) + at( krt, 2, i ) * at( uu, 2, j );
+         const double t2 = at( uu, i, 0 ) * at( krt, 0, j ) + at( uu, i, 1 ) * at( krt, 1, j ) + at( uu, i, 2 ) * at( krt, 2, j );
+         at( krr, i, j ) = at( krr, i, j ) - t + t1 - t2;
+      }
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j )
+         at( krt, i, j ) = at( krt, i, j ) + at( ktt, i, 0 ) * at( uu, 0, j ) + at( ktt, i, 1 ) * at( uu, 1, j ) + at( ktt, i, 2 ) * at( uu, 2, j );
+   std::memcpy( T.ktt, ktt, sizeof ktt );
+   std::memcpy( T.krt, krt, sizeof krt );
+   std::memcpy( T.krr, krr, sizeof krr );
+   // Dtt, Dtr, Drr with the coupling (difco.f:244-330).
+   std::memcpy( work, ktt, sizeof work );
+   inverse3( work, kinv );
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) {
+         double t = 0.0;
+         for ( int k = 0; k < 3; ++k )
+            t = t + at( krt, i, k ) * ( at( kinv, k, 0 ) * at( krt, j, 0 ) + at( kinv, k, 1 ) * at( krt, j, 1 ) + at( kinv, k, 2 ) * at( krt, j, 2 ) );
+         at( work, i, j ) = at( krr, i, j ) - t;
+      }
+   inverse3( work, T.drr );
+   if ( cg == 0 )
+      for ( int k = 0; k < 9; ++k ) T.drr[ k ] = 4.0 * T.drr[ k ] / 3.0;
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) {
+         double t = 0.0;
+         for ( int k = 0; k < 3; ++k )
+            t = t + at( kinv, i, k ) * ( at( krt, 0, k ) * at( T.drr, 0, j ) + at( krt, 1, k ) * at( T.drr, 1, j ) + at( krt, 2, k ) * at( T.drr, 2, j ) );
+         at( T.dtr, i, j ) = -t;
+         if ( cg == 0 ) at( T.dtr, i, j ) = 4.0 * at( T.dtr, i, j ) / 3.0;
+      }
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) {
+         double t = 0.0;
+         for ( int k = 0; k < 3; ++k )
+            t = t + at( kinv, i, k ) * ( at( krt, 0, k ) * at( T.dtr, j, 0 ) + at( krt, 1, k ) * at( T.dtr, j, 1 ) + at( krt, 2, k ) * at( T.dtr, j, 2 ) );
+         at( T.dtt, i, j ) = at( kinv, i, j ) - t;
+         if ( cg == 0 ) at( T.dtt, i, j ) = 4.0 * at( T.dtt, i, j ) / 3.0;
+      }
+   // Centre of diffusion (difco.f:344-401).
+   const double* drr = T.drr;
+   at( uu, 0, 0 ) = at( drr, 1, 1 ) + at( drr, 2, 2 );
+   at( uu, 1, 1 ) = at( drr, 0, 0 ) + at( drr, 2, 2 );
+   at( uu, 2, 2 ) = at( drr, 1, 1 ) + at( drr, 0, 0 );
+   at( uu, 0, 1 ) = -at( drr, 0, 1 );
+   at( uu, 0, 2 ) = -at( drr, 0, 2 );
+   at( uu, 2, 1 ) = -at( drr, 2, 1 );
+   at( uu, 1, 0 ) = at( uu, 0, 1 );
+   at( uu, 1, 2 ) = at( uu, 2, 1 );
+   at( uu, 2, 0 ) = at( uu, 0, 2 );
+   const double s1 = at( T.dtr, 2, 1 ) - at( T.dtr, 1, 2 );
+   const double s2 = at( T.dtr, 0, 2 ) - at( T.dtr, 2, 0 );
+   const double s3 = at( T.dtr, 1, 0 ) - at( T.dtr, 0, 1 );
+   inverse3( uu, aux );
+   for ( int j = 0; j < 3; ++j ) T.udr[ j ] = ( at( aux, j, 0 ) * s1 + at( aux, j, 1 ) * s2 + at( aux, j, 2 ) * s3 );
+   cross_matrix( T.udr, uu );
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j ) {
+         double t = 0.0;
+         for ( int k = 0; k < 3; ++k )
+            t = t + at( uu, i, k ) * at( drr, k, 0 ) * at( uu, 0, j ) + at( uu, i, k ) * at( drr, k, 1 ) * at( uu, 1, j ) +
+                at( uu, i, k ) * at( drr, k, 2 ) * at( uu, 2, j );
+         const double t1 = at( T.dtr, i, 0 ) * at( uu, 0, j ) + at( T.dtr, i, 1 ) * at( uu, 1, j ) + at( T.dtr, i, 2 ) * at( uu, 2, j );
+         const double t2 = -at( uu, i, 0 ) * at( T.dtr, j, 0 ) - at( uu, i, 1 ) * at( T.dtr, j, 1 ) - at( uu, i, 2 ) * at( T.dtr, j, 2 );
+         at( T.dtt, i, j ) = at( T.dtt, i, j ) - t + t1 + t2;
+      }
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j )
+         at( T.dtr, i, j ) = at( T.dtr, i, j ) - at( uu, i, 0 ) * at( drr, 0, j ) - at( uu, i, 1 ) * at( drr, 1, j ) -
+                             at( uu, i, 2 ) * at( drr, 2, j );
+}
+
+void visco_centre( const double* s, double d[ 3 ] ) {
+   double bb[ 9 ], bbi[ 9 ];
+   at( bb, 0, 0 ) = s[ 45 ] / 15.;
+   at( bb, 0, 1 ) = s[ 46 ] / 30.;
+   at( bb, 0, 2 ) = s[ 47 ] / 30.;
+   at( bb, 1, 0 ) = at( bb, 0, 1 );
+   at( bb, 1, 1 ) = s[ 48 ] / 15.;
+   at( bb, 1, 2 ) = s[ 49 ] / 30.;
+   at( bb, 2, 0 ) = at( bb, 0, 2 );
+   at( bb, 2, 1 ) = at( bb, 1, 2 );
+   at( bb, 2, 2 ) = s[ 50 ] / 15.;
+   const double c1 = s[ 51 ] / 30., c2 = s[ 52 ] / 30., c3 = s[ 53 ] / 30.;
+   inverse3( bb, bbi );
+   d[ 0 ] = at( bbi, 0, 0 ) * c1 + at( bbi, 0, 1 ) * c2 + at( bbi, 0, 2 ) * c3;
+   d[ 1 ] = at( bbi, 1, 0 ) * c1 + at( bbi, 1, 1 ) * c2 + at( bbi, 1, 2 ) * c3;
+   d[ 2 ] = at( bbi, 2, 0 ) * c1 + at( bbi, 2, 1 ) * c2 + at( bbi, 2, 2 ) * c3;
+}
+
+double visco_chiprime( const double* s, double cg ) {
+   const double pi = 3.1415926535897932385;
+   const double c112 = 0.5 * s[ 36 ], c213 = 0.5 * s[ 37 ], c323 = 0.5 * s[ 38 ];
+   double chip = 8 * pi * ( 0.2 * ( c112 + c213 + c323 ) + ( s[ 39 ] - 2 * s[ 40 ] + s[ 41 ] + s[ 42 ] + s[ 43 ] - 2 * s[ 44 ] ) / 15.0 );
+   if ( cg == 0 ) chip = 0.75 * chip;
+   return chip;
+}
+
+void output_values( Transport& T ) {
+   const double temperature = 293.15, eta = 0.01002, pi = 3.141592653;
+   const double tunit = ( 1.38066E-16 ) * temperature * 1.0E8 / ( 8.0 * eta * pi );
+   const double runit = tunit * 1.0e16;
+   double a[ 9 ];
+   std::memcpy( a, T.drr, sizeof a );
+   eigen3( a, T.drr_eig );
+   T.dr = runit * ( ( T.drr_eig[ 0 ] + T.drr_eig[ 1 ] + T.drr_eig[ 2 ] ) / 3. );
+   std::memcpy( a, T.dtt, sizeof a );
+   eigen3( a, T.dtt_eig );
+   T.dtrans = tunit * ( ( T.dtt_eig[ 0 ] + T.dtt_eig[ 1 ] + T.dtt_eig[ 2 ] ) / 3. );
+}
+
+}  // namespace bhg

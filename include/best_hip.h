@@ -1,0 +1,145 @@
+/* best_hip.h -- BEST's boundary-element Stokes solve on one MI355X GPU.
+ *
+ * Part 1 is the C ABI of the reference's H-matrix library, hmatrix/best_hmatrix.h:26-73, with
+ * the same nine names, arguments, layouts and error contract, so that BEST links this library
+ * where it linked $(HM_OBJ) (Makefile:54-62) or best/hmstub.f90, and best/besolve.f90:48-89
+ * binds it unchanged.  Behind it the whole dense path runs on the GPU: the GR entries
+ * (best/tgetgr.f, best/triangleint.f), the deflation, an LU on the matrix cores and the solves.
+ *
+ * Part 2 is the handle-based API the Python layer, the tests and the benchmark use: one solver
+ * per handle, a mesh and its factors resident in device memory, and the whole mesh solve of
+ * best/best.f:261-498 (without file I/O) in one call.
+ *
+ * Layouts follow BEST: vertex and incenter arrays are NAR x 3 column-major (x[c*n + i]);
+ * matrices and right-hand-side blocks are column-major; unknown 3*i + a is component a on
+ * triangle i (0-based).  All pointers are host pointers.
+ */
+#ifndef BEST_HIP_H
+#define BEST_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ Part 1: hmatrix ABI */
+
+/* hmatrix/best_hmatrix.h:27 (stand-in best/hmstub.f90:7-11).  Returns 1. */
+int bhm_available( void );
+
+/* hmatrix/best_hmatrix.h:35-36.  The mesh as HMPREP passes it (best/besolve.f90:164): vertices
+ * after the centroid shift, areas after the area correction, afac, cg.  Computes the incenters,
+ * self blocks and the mean diagonal on the GPU (device BEST_HIP_DEVICE, default 0).  threads is
+ * ignored.  Returns 0, or nonzero after queuing a message. */
+int bhm_create( int n, const double* v1, const double* v2, const double* v3, const double* area, double afac,
+                double cg, int threads );
+
+/* hmatrix/best_hmatrix.h:39: TGETGR's X, n x 3 column-major. */
+void bhm_incenters( double* x );
+
+/* hmatrix/best_hmatrix.h:43: GR(i0 : i0+m-1, j0 : j0+nc-1) before deflation, column-major. */
+void bhm_entries( int i0, int m, int j0, int nc, double* a, int lda );
+
+/* hmatrix/best_hmatrix.h:46: the mean diagonal entry of GR in BEST's summation order. */
+double bhm_diag_mean( void );
+
+/* hmatrix/best_hmatrix.h:62-63.  Assembles GR + sum_k gdef[k] xmu_k xmu_k^T densely on the GPU
+ * and factors it: single = 1 (or BEST_HIP_LU unset) FP32 LU on MFMA with FP64 iterative
+ * refinement in bhm_solve; BEST_HIP_LU=fp64 and single = 0: FP64 LU.  eps, direct, eps_pre and
+ * nrhs are accepted for the ABI; the dense solve is exact to rounding.  Returns nonzero (with a
+ * message) when the matrix does not fit in device memory. */
+int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* gdef, const int* comp, const double* xmu,
+               int single, int nrhs );
+
+/* hmatrix/best_hmatrix.h:67: b (3n x nrhs, column-major) in, solutions out. */
+int bhm_solve( int nrhs, double* b );
+
+/* hmatrix/best_hmatrix.h:71: the next queued message (no NUL), or -1. */
+int bhm_log_line( char* buf, int len );
+
+/* hmatrix/best_hmatrix.h:74: frees everything. */
+void bhm_destroy( void );
+
+/* ------------------------------------------------------------------ Part 2: solver handles */
+
+typedef struct bhg_solver bhg_solver;
+
+/* The values BEST writes to its .be file (best/output.f) and the tensors behind them; 3x3
+ * tensors column-major as in best/difco.f. */
+typedef struct bhg_result {
+   double dtt[ 9 ], dtr[ 9 ], drr[ 9 ]; /* diffusion tensors at the centre of diffusion (1/A, 1/A^2, 1/A^3) */
+   double urc[ 3 ], udr[ 3 ];           /* centre of resistance - centroid, centre of diffusion - resistance */
+   double ktt[ 9 ], krt[ 9 ], krr[ 9 ]; /* friction tensors at the centre of resistance */
+   double drr_eig[ 3 ], dtt_eig[ 3 ];   /* JACOBI eigenvalues */
+   double dr;                           /* 1/3 Trace DRR(EXACT), 1/s */
+   double dtrans;                       /* 1/3 Trace DTT(EXACT), cm^2/s */
+   double chiprime;                     /* CHI * VOLUME, A^3 (with ivisco) */
+   double chi;                          /* CHIPRIME / TRUEVOL (with the area correction) */
+   double dvisc[ 3 ];                   /* centre of viscosity - centroid (icov), else 0 */
+   double afac, totarea, centroid[ 3 ], diagm;
+   int ncomp, nopen, ndef;
+} bhg_result;
+
+/* Timings (ms, HIP events on the solver's stream) and diagnostics of the last call. */
+typedef struct bhg_stats {
+   double t_prep, t_assemble, t_factor, t_solve, t_reduce, t_total;
+   int ir_iterations;    /* refinement steps of the FP32 path (0 with FP64 LU) */
+   double ir_correction; /* last max_j |dx_j| / |x_j| */
+   int pivoted;          /* 1 if the factorization needed row exchanges */
+   int npad;             /* padded order of the matrix */
+} bhg_stats;
+
+bhg_solver* bhg_create( int device );
+void bhg_destroy( bhg_solver* s );
+const char* bhg_last_error( bhg_solver* s );
+int bhg_log_line( bhg_solver* s, char* buf, int len );
+
+/* COMPON (best/compon.f90:19-115): icomp[n] in 1..ncomp, and the open-edge count. */
+int bhg_compon( int n, const double* v1, const double* v2, const double* v3, int* icomp, int* ncomp, int* nopen );
+
+/* A mesh as BEST reads it (best/best.f:189-220, before the centroid shift), made resident.  icomp
+ * may be NULL (COMPON is run here). */
+int bhg_set_mesh( bhg_solver* s, int n, const double* v1, const double* v2, const double* v3, double truearea,
+                  double truevol, const int* icomp );
+
+/* One mesh solve on the resident mesh: best/best.f:261-498 without file I/O (prep, TGETGR,
+ * deflation, DGETRF, DIFCO, VISCO, the .be values).  iarea = 1 applies the area correction
+ * (BEST's default, -na gives 0); cg = 1, or 0 with -p; ivisco = 1 solves the strain flows (-v);
+ * icov = 1 moves to the centre of viscosity (-vc); precision = 64 (FP64 LU) or 32 (FP32 LU on
+ * MFMA + FP64 refinement). */
+int bhg_solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int precision, bhg_result* out );
+
+/* The last solutions (3n x nrhs, columns: 3 translations, 3 rotations, 5 strain flows). */
+int bhg_get_solution( bhg_solver* s, int nrhs, double* out, long ldo );
+int bhg_get_stats( bhg_solver* s, bhg_stats* out );
+int bhg_set_refinement( bhg_solver* s, double tol, int maxit );
+
+/* Launch profiling of bhg_solve_mesh: with on = 1 every launch of the timed kernel classes is
+ * bracketed by HIP events on the solver's stream.  bhg_get_profile returns, per class (0 pair
+ * assembly, 1 FP64 GEMM, 2 FP32 GEMM, 3 diagonal blocks, 4 triangular solves, 5 refinement
+ * residuals), the summed duration (ms), algorithmic flops and launch count since profiling was
+ * turned on. */
+int bhg_set_profiling( bhg_solver* s, int on );
+int bhg_get_profile( bhg_solver* s, double* ms, double* flops, long* count );
+
+/* The prepared mesh of bhm_create (shifted vertices, corrected areas). */
+int bhg_load_prepared( bhg_solver* s, int n, const double* v1, const double* v2, const double* v3, const double* area,
+                       double afac, double cg );
+int bhg_incenters( bhg_solver* s, double* x );
+int bhg_entries( bhg_solver* s, int i0, int m, int j0, int nc, double* a, long lda );
+double bhg_diag_mean( bhg_solver* s );
+
+/* TGETGR twin (best/tgetgr.f:1): X and GR (no deflation) of the prepared mesh into host memory. */
+int bhg_tgetgr( bhg_solver* s, double* x, double* gr, long ldgr );
+
+/* best/best.f:384-468 on the GPU: GR + deflation assembled and factored (DGETRF twin). gdef NULL:
+ * the weights are computed on the device in BEST's order. */
+int bhg_factor_deflated( bhg_solver* s, int ndef, const double* gdef, const int* comp, const double* xmu, int precision );
+
+/* BESOLV twin (best/besolve.f90:95-134): b (3n x nrhs, leading dimension ldb) in place. */
+int bhg_besolv( bhg_solver* s, int nrhs, double* b, long ldb );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
