@@ -67,15 +67,15 @@ struct LaunchProfiler {
       }
       return pool[ used++ ];
    }
-   void begin( int k, double f ) {
+   void begin( int k, double f, hipStream_t s ) {
       if ( !on ) return;
-      (void)hipEventRecord( next(), st );
+      (void)hipEventRecord( next(), s ? s : st );
       cls.push_back( k );
       flops.push_back( f );
    }
-   void end() {
+   void end( hipStream_t s ) {
       if ( !on ) return;
-      (void)hipEventRecord( next(), st );
+      (void)hipEventRecord( next(), s ? s : st );
    }
    // After the stream has been synchronized.
    void collect() {
@@ -99,11 +99,12 @@ struct LaunchProfiler {
 extern thread_local LaunchProfiler* g_prof;
 
 struct ProfScope {
-   explicit ProfScope( int k, double f ) {
-      if ( g_prof ) g_prof->begin( k, f );
+   hipStream_t s;
+   ProfScope( int k, double f, hipStream_t stream = nullptr ) : s( stream ) {
+      if ( g_prof ) g_prof->begin( k, f, s );
    }
    ~ProfScope() {
-      if ( g_prof ) g_prof->end();
+      if ( g_prof ) g_prof->end( s );
    }
 };
 

@@ -17,6 +17,8 @@
 // column-by-column partial pivoting (first row of largest |a|, as IDAMAX chooses).
 //
 // Matrices are column-major, padded to a multiple of 128 with an identity block.
+#include <algorithm>
+
 #include "bh_common.h"
 
 namespace bhg {
@@ -50,10 +52,22 @@ __device__ inline T tabs( T v ) {
 // registers while the current one is multiplied.  C may alias A (when N == 128: each workgroup
 // reads its whole row strip before it writes) or B (when M == 128).  check != nullptr: set
 // *check when any |C| > 1 (the pivot test of the panel below the diagonal block).
+// Batched over blockIdx.z: batch z = zdiv * q + t offsets A by q sa.hi + t sa.lo, and likewise B, C.
+struct BatchStride {
+   long hi = 0, lo = 0;
+};
+
 template <typename T, int BK>
 __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
-                                                T beta, T* C, long ldc, int* check ) {
+                                                T beta, T* C, long ldc, int* check, int zdiv, BatchStride sa,
+                                                BatchStride sb, BatchStride sc ) {
    constexpr int BM = 128, BN = 128, PAD = 4;
+   {
+      const long q = blockIdx.z / zdiv, t = blockIdx.z % zdiv;
+      A += q * sa.hi + t * sa.lo;
+      B += q * sb.hi + t * sb.lo;
+      C += q * sc.hi + t * sc.lo;
+   }
    __shared__ T As[ BK ][ BM + PAD ];
    __shared__ T Bs[ BK ][ BN + PAD ];
    using v4 = typename Mfma<T>::v4;
@@ -133,13 +147,15 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
 // (upper) as two dense 128 x 128 column-major matrices.
 //
 // One workgroup of 256 threads; thread (ty, tx) of a 16 x 16 grid keeps the 8 x 8 tile at rows
-// 8 ty.., columns 8 tx.. in registers.  Each elimination step broadcasts pivot row and column
-// through LDS (double-buffered, so one barrier per step) and every thread applies the rank-1
-// update to its own tile.  The multipliers are a_ij * (1 / a_jj), as DGETF2 scales.
-// The inversions run in place in the same loop: U^-1 by rows from the bottom (step s works on row
-// 127 - s: X(j,:) = Y(j,:) / u_jj, then Y(i,:) -= u_ij X(j,:) for i < j, where a position still
-// holding u_ic becomes X's entry when column c is reached) and L^-1 by rows from the top (unit
-// diagonal).  The two touch disjoint halves of the tile grid.
+// 8 ty.., columns 8 tx.. in registers.  The step loop runs over tile columns jb and, unrolled,
+// over the offset jo inside the tile, so every register index is a compile-time constant.  Each
+// step broadcasts pivot row and column through LDS (double-buffered: one barrier per step) and
+// applies the rank-1 update to the whole tile with multipliers and pivot-row entries set to zero
+// outside the active part, so the update itself has no per-element conditions.  The multipliers
+// are a_ij * (1 / a_jj), as DGETF2 scales.  The inversions follow in the same way: U^-1 by rows
+// from the bottom (X(j,:) = Y(j,:) / u_jj, then Y(i,:) -= u_ij X(j,:) for i < j, where a position
+// still holding u_ic becomes X's entry when column c is reached) and L^-1 by rows from the top
+// (unit diagonal).  The two touch disjoint halves of the block.
 template <typename T>
 __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T* __restrict__ inv, int* check ) {
    constexpr int NB = kNB, RB = 8;
@@ -153,41 +169,42 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
       for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = A[ (long)( c0 + jj ) * lda + r0 + ii ];
    if ( mode == 0 ) {
       bool big = false;
-      for ( int j = 0; j < NB; ++j ) {
-         const int bsel = j & 1, jb = j >> 3, jo = j & 7;
-         if ( tx == jb ) {
+      for ( int jb = 0; jb < NB / RB; ++jb ) {
 #pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) colb[ bsel ][ r0 + ii ] = a[ ii ][ jo ];
-         }
-         if ( ty == jb ) {
+         for ( int jo = 0; jo < RB; ++jo ) {
+            const int j = jb * RB + jo, bsel = jo & 1;
+            if ( tx == jb ) {
 #pragma unroll
-            for ( int jj = 0; jj < RB; ++jj ) rowb[ bsel ][ c0 + jj ] = a[ jo ][ jj ];
-         }
-         __syncthreads();
-         if ( r0 + RB - 1 <= j ) continue;  // no rows below the pivot in this tile
-         const T piv = rowb[ bsel ][ j ];
-         const T rp = T( 1 ) / piv;
-         T l[ RB ];
+               for ( int ii = 0; ii < RB; ++ii ) colb[ bsel ][ r0 + ii ] = a[ ii ][ jo ];
+            }
+            if ( ty == jb ) {
 #pragma unroll
-         for ( int ii = 0; ii < RB; ++ii ) {
-            const int i = r0 + ii;
-            const T v = colb[ bsel ][ i ];
-            const bool below = i > j;
-            if ( below ) big |= !( tabs( v ) <= tabs( piv ) );
-            l[ ii ] = below ? v * rp : T( 0 );
-         }
-         if ( tx == jb ) {
+               for ( int jj = 0; jj < RB; ++jj ) rowb[ bsel ][ c0 + jj ] = a[ jo ][ jj ];
+            }
+            __syncthreads();
+            if ( r0 + RB - 1 > j ) {  // this tile has rows below the pivot
+               const T piv = rowb[ bsel ][ j ];
+               const T rp = T( 1 ) / piv;
+               T l[ RB ], u[ RB ];
 #pragma unroll
-            for ( int ii = 0; ii < RB; ++ii )
-               if ( r0 + ii > j ) a[ ii ][ jo ] = l[ ii ];
-         }
-         if ( c0 + RB - 1 <= j ) continue;
+               for ( int ii = 0; ii < RB; ++ii ) {
+                  const T v = colb[ bsel ][ r0 + ii ];
+                  const bool below = r0 + ii > j;
+                  big |= below && !( tabs( v ) <= tabs( piv ) );
+                  l[ ii ] = below ? v * rp : T( 0 );
+               }
 #pragma unroll
-         for ( int jj = 0; jj < RB; ++jj ) {
-            if ( c0 + jj <= j ) continue;
-            const T u = rowb[ bsel ][ c0 + jj ];
+               for ( int jj = 0; jj < RB; ++jj ) u[ jj ] = c0 + jj > j ? rowb[ bsel ][ c0 + jj ] : T( 0 );
 #pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - l[ ii ] * u;
+               for ( int jj = 0; jj < RB; ++jj )
+#pragma unroll
+                  for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - l[ ii ] * u[ jj ];
+               if ( tx == jb ) {
+#pragma unroll
+                  for ( int ii = 0; ii < RB; ++ii )
+                     if ( r0 + ii > j ) a[ ii ][ jo ] = l[ ii ];
+               }
+            }
          }
       }
       if ( big ) atomicOr( check, 1 );
@@ -197,67 +214,72 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
          for ( int ii = 0; ii < RB; ++ii ) A[ (long)( c0 + jj ) * lda + r0 + ii ] = a[ ii ][ jj ];
    }
    __syncthreads();
-   for ( int st = 0; st < NB; ++st ) {
-      const int bsel = st & 1;
-      const int ju = NB - 1 - st, jl = st;
-      // U^-1 step on row ju: broadcast u_{i,ju} (i < ju) and Y(ju, c >= ju) with u_{ju,ju}.
-      if ( tx == ( ju >> 3 ) ) {
+   for ( int sb = 0; sb < NB / RB; ++sb ) {
 #pragma unroll
-         for ( int ii = 0; ii < RB; ++ii ) colb[ bsel ][ r0 + ii ] = a[ ii ][ ju & 7 ];
-      }
-      if ( ty == ( ju >> 3 ) ) {
+      for ( int so = 0; so < RB; ++so ) {
+         const int bsel = so & 1;
+         // U^-1 works on row ju = 127 - (8 sb + so): tile row/column offset 7 - so, tile index 15 - sb.
+         const int ju = NB - 1 - ( sb * RB + so ), uo = RB - 1 - so, ub = NB / RB - 1 - sb;
+         // L^-1 works on row jl = 8 sb + so: offset so, tile index sb.
+         const int jl = sb * RB + so;
+         if ( tx == ub ) {
 #pragma unroll
-         for ( int jj = 0; jj < RB; ++jj ) rowb[ bsel ][ c0 + jj ] = a[ ju & 7 ][ jj ];
-      }
-      // L^-1 step on row jl: broadcast l_{i,jl} (i > jl) and X(jl, c < jl).
-      if ( tx == ( jl >> 3 ) ) {
-#pragma unroll
-         for ( int ii = 0; ii < RB; ++ii ) colL[ bsel ][ r0 + ii ] = a[ ii ][ jl & 7 ];
-      }
-      if ( ty == ( jl >> 3 ) ) {
-#pragma unroll
-         for ( int jj = 0; jj < RB; ++jj ) rowL[ bsel ][ c0 + jj ] = a[ jl & 7 ][ jj ];
-      }
-      __syncthreads();
-      const T d = T( 1 ) / rowb[ bsel ][ ju ];
-      // row ju itself: X(ju, ju) = d, X(ju, c) = Y(ju, c) d for c > ju
-      if ( ty == ( ju >> 3 ) ) {
-#pragma unroll
-         for ( int jj = 0; jj < RB; ++jj ) {
-            const int c = c0 + jj;
-            if ( c == ju ) a[ ju & 7 ][ jj ] = d;
-            else if ( c > ju ) a[ ju & 7 ][ jj ] = a[ ju & 7 ][ jj ] * d;
+            for ( int ii = 0; ii < RB; ++ii ) colb[ bsel ][ r0 + ii ] = a[ ii ][ uo ];
          }
-      }
-      // rows i < ju: position (i, ju) <- -u_{i,ju} d; (i, c > ju) -= u_{i,ju} X(ju, c)
-      if ( r0 < ju && c0 + RB - 1 >= ju ) {
+         if ( ty == ub ) {
 #pragma unroll
-         for ( int jj = 0; jj < RB; ++jj ) {
-            const int c = c0 + jj;
-            if ( c < ju ) continue;
-            const T xj = c == ju ? d : rowb[ bsel ][ c ] * d;
+            for ( int jj = 0; jj < RB; ++jj ) rowb[ bsel ][ c0 + jj ] = a[ uo ][ jj ];
+         }
+         if ( tx == sb ) {
 #pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) {
-               const int i = r0 + ii;
-               if ( i >= ju ) continue;
-               const T uij = colb[ bsel ][ i ];
-               a[ ii ][ jj ] = c == ju ? -uij * d : a[ ii ][ jj ] - uij * xj;
+            for ( int ii = 0; ii < RB; ++ii ) colL[ bsel ][ r0 + ii ] = a[ ii ][ so ];
+         }
+         if ( ty == sb ) {
+#pragma unroll
+            for ( int jj = 0; jj < RB; ++jj ) rowL[ bsel ][ c0 + jj ] = a[ so ][ jj ];
+         }
+         __syncthreads();
+         const T d = T( 1 ) / rowb[ bsel ][ ju ];
+         // U^-1: rows i < ju get (i, ju) <- -u_{i,ju} d and (i, c > ju) -= u_{i,ju} X(ju, c).
+         if ( r0 < ju ) {
+            T uc[ RB ], xr[ RB ];
+#pragma unroll
+            for ( int ii = 0; ii < RB; ++ii ) uc[ ii ] = r0 + ii < ju ? colb[ bsel ][ r0 + ii ] : T( 0 );
+#pragma unroll
+            for ( int jj = 0; jj < RB; ++jj ) xr[ jj ] = c0 + jj > ju ? rowb[ bsel ][ c0 + jj ] * d : T( 0 );
+#pragma unroll
+            for ( int jj = 0; jj < RB; ++jj )
+#pragma unroll
+               for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - uc[ ii ] * xr[ jj ];
+            if ( tx == ub ) {
+#pragma unroll
+               for ( int ii = 0; ii < RB; ++ii )
+                  if ( r0 + ii < ju ) a[ ii ][ uo ] = -uc[ ii ] * d;
             }
          }
-      }
-      // rows i > jl: position (i, jl) <- -l_{i,jl}; (i, c < jl) -= l_{i,jl} X(jl, c)
-      if ( r0 + RB - 1 > jl && c0 <= jl ) {
+         // row ju itself: X(ju, ju) = d, X(ju, c > ju) = Y(ju, c) d
+         if ( ty == ub ) {
 #pragma unroll
-         for ( int jj = 0; jj < RB; ++jj ) {
-            const int c = c0 + jj;
-            if ( c > jl ) continue;
-            const T xj = rowL[ bsel ][ c ];
+            for ( int jj = 0; jj < RB; ++jj ) {
+               const int c = c0 + jj;
+               a[ uo ][ jj ] = c == ju ? d : ( c > ju ? a[ uo ][ jj ] * d : a[ uo ][ jj ] );
+            }
+         }
+         // L^-1: rows i > jl get (i, jl) <- -l_{i,jl} and (i, c < jl) -= l_{i,jl} X(jl, c).
+         if ( r0 + RB - 1 > jl ) {
+            T lc[ RB ], xl[ RB ];
 #pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) {
-               const int i = r0 + ii;
-               if ( i <= jl ) continue;
-               const T lij = colL[ bsel ][ i ];
-               a[ ii ][ jj ] = c == jl ? -lij : a[ ii ][ jj ] - lij * xj;
+            for ( int ii = 0; ii < RB; ++ii ) lc[ ii ] = r0 + ii > jl ? colL[ bsel ][ r0 + ii ] : T( 0 );
+#pragma unroll
+            for ( int jj = 0; jj < RB; ++jj ) xl[ jj ] = c0 + jj < jl ? rowL[ bsel ][ c0 + jj ] : T( 0 );
+#pragma unroll
+            for ( int jj = 0; jj < RB; ++jj )
+#pragma unroll
+               for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - lc[ ii ] * xl[ jj ];
+            if ( tx == sb ) {
+#pragma unroll
+               for ( int ii = 0; ii < RB; ++ii )
+                  if ( r0 + ii > jl ) a[ ii ][ so ] = -lc[ ii ];
             }
          }
       }
@@ -343,90 +365,133 @@ __global__ void __launch_bounds__( 256 ) k_piv_update( int npad, int j, int kend
 
 // ------------------------------------------------------------------ solves
 
-// One launch solves L Y = B (lower = 1: unit lower factor, block rows top to bottom) or U X = Y
-// (lower = 0: upper factor, bottom to top) for the nrhs <= 16 columns of b (npad x nrhs, in
-// place), using the factor panels of LU and the inverted diagonal blocks inv.
-//
-// Workgroup t (in the order workgroups start, from the ticket counter) owns the t-th block row
-// in solve order.  It accumulates the products with every block of the solution that precedes
-// it as soon as that block's flag is raised, then multiplies by its inverted diagonal block,
-// publishes its block (agent-scope release) and raises its own flag.  A workgroup only waits for
-// workgroups that drew earlier tickets, which are already running, so the chain always advances;
-// each wait is bounded (spin_limit), after which *err is set and the kernel ends normally.
-// done[] and *ticket must be zero at launch.
+// Partial products of the triangular solves on the matrix cores: P[kc] = A (M x K) X for the
+// kc-th slice of kper columns of K, with A column-major and X, P row-interleaved (element (i, r)
+// at 16 i + r, 16 right-hand-side columns).  Workgroup (row block of 64, slice kc); its four waves
+// split the slice, each running 16 x 16 x 4 MFMAs over 4 row tiles (the X fragment of a 4-column
+// step is one coalesced load, 64 consecutive values), and add their tiles in LDS.  Splitting K
+// keeps enough waves resident to cover the memory latency; k_skinny_reduce adds the slices in a
+// fixed order.
 template <typename T>
-__global__ void __launch_bounds__( 256 ) k_trsm_chain( int nblk, int lower, int nrhs, const T* __restrict__ LU, long lda,
-                                                      const T* __restrict__ inv, T* b, long ldb, int* ticket, int* done,
-                                                      int* err, int spin_limit ) {
-   constexpr int NB = kNB;
-   __shared__ T xs[ NB ][ 17 ];
-   __shared__ T red[ 2 ][ NB ][ 17 ];
-   __shared__ int s_t;
-   const int tid = threadIdx.x;
-   if ( tid == 0 ) s_t = atomicAdd( ticket, 1 );
-   __syncthreads();
-   const int t = s_t;
-   const int w = lower ? t : nblk - 1 - t;
-   const int i = tid & ( NB - 1 ), kh = tid >> 7;
-   T acc[ 16 ];
+__global__ void __launch_bounds__( 256 ) k_skinny_part( int M, int K, int kper, const T* __restrict__ A, long lda,
+                                                       const T* __restrict__ X, T* __restrict__ P ) {
+   using v4 = typename Mfma<T>::v4;
+   constexpr int PF = 4;
+   __shared__ T red[ 3 ][ 64 * 16 ];
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   const int row0 = blockIdx.x * 64;
+   const int kc = blockIdx.y;
+   const int kw = kper / 4;
+   const int kb = kc * kper + w * kw;
+   const int kn = max( 0, min( kw, K - kb ) );
+   v4 acc[ 4 ];
 #pragma unroll
-   for ( int r = 0; r < 16; ++r ) acc[ r ] = T( 0 );
-   for ( int q = 0; q < t; ++q ) {
-      const int j = lower ? q : nblk - 1 - q;
-      if ( tid == 0 ) {
-         int spins = 0;
-         while ( __hip_atomic_load( &done[ j ], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT ) == 0 ) {
-            __builtin_amdgcn_s_sleep( 1 );
-            if ( ++spins > spin_limit ) {
-               atomicOr( err, 1 );
-               break;
-            }
+   for ( int t = 0; t < 4; ++t ) acc[ t ] = v4{ 0, 0, 0, 0 };
+   const int li = lane & 15, lk = lane >> 4;
+   const T* a = A + (long)( kb + lk ) * lda + row0 + li;
+   const T* x = X + (long)( kb + lk ) * 16 + li;
+   for ( int k = 0; k < kn; k += 4 * PF ) {
+      T fa[ PF ][ 4 ], fb[ PF ];
+#pragma unroll
+      for ( int u = 0; u < PF; ++u ) {
+         const bool in = k + 4 * u < kn;
+         const long off = (long)( k + 4 * u ) * lda;
+#pragma unroll
+         for ( int t = 0; t < 4; ++t ) fa[ u ][ t ] = in ? a[ off + 16 * t ] : T( 0 );
+         fb[ u ] = in ? x[ (long)( k + 4 * u ) * 16 ] : T( 0 );
+      }
+#pragma unroll
+      for ( int u = 0; u < PF; ++u )
+#pragma unroll
+         for ( int t = 0; t < 4; ++t ) acc[ t ] = Mfma<T>::run( fa[ u ][ t ], fb[ u ], acc[ t ] );
+   }
+   // tile t, accumulator element r of this lane: row 16 t + (FP32: 4 (lane/16) + r, FP64:
+   // lane/16 + 4 r), right-hand side lane % 16
+   auto rowof = [&]( int t, int r ) { return 16 * t + ( sizeof( T ) == 8 ? lk + 4 * r : 4 * lk + r ); };
+   if ( w > 0 ) {
+#pragma unroll
+      for ( int t = 0; t < 4; ++t )
+#pragma unroll
+         for ( int r = 0; r < 4; ++r ) red[ w - 1 ][ rowof( t, r ) * 16 + li ] = acc[ t ][ r ];
+   }
+   __syncthreads();
+   if ( w == 0 ) {
+      T* out = P + (long)kc * M * 16 + (long)row0 * 16;
+#pragma unroll
+      for ( int t = 0; t < 4; ++t )
+#pragma unroll
+         for ( int r = 0; r < 4; ++r ) {
+            const int e = rowof( t, r ) * 16 + li;
+            out[ e ] = ( acc[ t ][ r ] + red[ 0 ][ e ] ) + ( red[ 1 ][ e ] + red[ 2 ][ e ] );
          }
-      }
-      __syncthreads();
-      for ( int idx = tid; idx < NB * nrhs; idx += 256 ) {
-         const int k = idx & ( NB - 1 ), r = idx >> 7;
-         xs[ k ][ r ] = __hip_atomic_load( &b[ (long)r * ldb + (long)j * NB + k ], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
-      }
-      __syncthreads();
-      const T* Lp = LU + (long)( j * NB + kh * 64 ) * lda + (long)w * NB + i;
-#pragma unroll 8
-      for ( int k = 0; k < 64; ++k ) {
-         const T l = Lp[ (long)k * lda ];
-#pragma unroll
-         for ( int r = 0; r < 16; ++r ) acc[ r ] = acc[ r ] + l * xs[ kh * 64 + k ][ r ];
-      }
-      __syncthreads();
    }
-   // b_w - sum, then the diagonal block's inverse.
-#pragma unroll
-   for ( int r = 0; r < 16; ++r ) red[ kh ][ i ][ r ] = acc[ r ];
-   __syncthreads();
-   for ( int idx = tid; idx < NB * nrhs; idx += 256 ) {
-      const int k = idx & ( NB - 1 ), r = idx >> 7;
-      xs[ k ][ r ] = b[ (long)r * ldb + (long)w * NB + k ] - ( red[ 0 ][ k ][ r ] + red[ 1 ][ k ][ r ] );
-   }
-   __syncthreads();
-   const T* M = inv + (long)w * 2 * NB * NB + ( lower ? 0 : NB * NB );
-   T out[ 8 ];
-#pragma unroll
-   for ( int r = 0; r < 8; ++r ) out[ r ] = T( 0 );
-   const int rb = kh * 8;
-#pragma unroll 8
-   for ( int k = 0; k < NB; ++k ) {
-      const T m = M[ k * NB + i ];
-#pragma unroll
-      for ( int r = 0; r < 8; ++r ) out[ r ] = out[ r ] + m * xs[ k ][ rb + r ];
-   }
-   for ( int r = 0; r < 8; ++r )
-      if ( rb + r < nrhs ) b[ (long)( rb + r ) * ldb + (long)w * NB + i ] = out[ r ];
-   __threadfence();
-   __syncthreads();
-   if ( tid == 0 ) __hip_atomic_store( &done[ w ], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
 }
 
-__global__ void k_zero_ints( int n, int* p ) {
-   for ( int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x ) p[ i ] = 0;
+// Y = beta Y + alpha sum_kc P[kc] (row-interleaved, M x 16), the slices added in order.
+template <typename T>
+__global__ void __launch_bounds__( 256 ) k_skinny_reduce( long n16, int nkc, T alpha, const T* __restrict__ P, T beta,
+                                                         T* __restrict__ Y ) {
+   for ( long e = blockIdx.x * 256L + threadIdx.x; e < n16; e += (long)gridDim.x * 256 ) {
+      T s = P[ e ];
+      for ( int c = 1; c < nkc; ++c ) s = s + P[ (long)c * n16 + e ];
+      Y[ e ] = beta == T( 0 ) ? alpha * s : beta * Y[ e ] + alpha * s;
+   }
+}
+
+// Column-major (ld) <-> row-interleaved (16 per row) blocks of npad rows; columns >= nrhs are zero.
+template <typename S, typename D>
+__global__ void k_to_interleaved( int npad, int nrhs, const S* __restrict__ b, long ldb, D* __restrict__ v ) {
+   for ( long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)npad * 16; e += (long)gridDim.x * blockDim.x ) {
+      const long i = e >> 4;
+      const int r = (int)( e & 15 );
+      v[ e ] = r < nrhs ? (D)b[ (long)r * ldb + i ] : D( 0 );
+   }
+}
+
+template <typename S, typename D>
+__global__ void k_from_interleaved( int n, int nrhs, const S* __restrict__ v, D* __restrict__ b, long ldb ) {
+   for ( long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * nrhs; e += (long)gridDim.x * blockDim.x ) {
+      const long i = e % n;
+      const int r = (int)( e / n );
+      b[ (long)r * ldb + i ] = (D)v[ i * 16 + r ];
+   }
+}
+
+// Row interchanges of the factorization applied in order to a row-interleaved block.
+template <typename T>
+__global__ void k_apply_ipiv_il( int n, const int* __restrict__ ipiv, T* v ) {
+   const int r = threadIdx.x;
+   if ( r >= 16 ) return;
+   for ( int k = 0; k < n; ++k ) {
+      const int p = ipiv[ k ] - 1;
+      if ( p != k ) {
+         const T t = v[ (long)k * 16 + r ];
+         v[ (long)k * 16 + r ] = v[ (long)p * 16 + r ];
+         v[ (long)p * 16 + r ] = t;
+      }
+   }
+}
+
+// The diagonal blocks of the super-block inverses: XL and XU hold one S x S matrix per super-block
+// q (columns q S .. q S + S - 1, leading dimension S); this writes the 128 x 128 inverses of the
+// factorization on their diagonals and zeros elsewhere.
+template <typename T>
+__global__ void k_super_init( int S, int nsb, const T* __restrict__ inv, T* __restrict__ XL, T* __restrict__ XU ) {
+   const long total = (long)S * S * nsb;
+   for ( long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x ) {
+      const long q = e / ( (long)S * S );
+      const int rem = (int)( e % ( (long)S * S ) );
+      const int c = rem / S, r = rem % S;
+      T l = T( 0 ), u = T( 0 );
+      if ( r / kNB == c / kNB ) {
+         const long k = q * ( S / kNB ) + c / kNB;
+         const T* blk = inv + k * 2 * kNB * kNB;
+         l = blk[ ( c % kNB ) * kNB + r % kNB ];
+         u = blk[ kNB * kNB + ( c % kNB ) * kNB + r % kNB ];
+      }
+      XL[ e ] = l;
+      XU[ e ] = u;
+   }
 }
 
 // Row interchanges of the factorization applied to b, in order (DLASWP forward).
@@ -448,30 +513,46 @@ __global__ void k_apply_ipiv( int n, int nrhs, const int* __restrict__ ipiv, T* 
 // ------------------------------------------------------------------ refinement
 
 // Partial products of A x over column chunks: part[c][rhs][row] = sum_{j in chunk c} A(row, j) x(j, rhs).
+// Each thread carries four rows (row0 + 256 m) so every broadcast read of x from LDS feeds four
+// loads of A.
 __global__ void __launch_bounds__( 256 ) k_resid_part( int npad, int chunk, int nrhs, const double* __restrict__ A,
                                                       long lda, const double* __restrict__ x, long ldx,
                                                       double* __restrict__ part ) {
-   extern __shared__ double xs[];  // chunk x 16
+   extern __shared__ __attribute__( ( aligned( 16 ) ) ) double xs[];  // chunk x 16
    const int c0 = blockIdx.y * chunk;
-   for ( int idx = threadIdx.x; idx < chunk * nrhs; idx += 256 ) {
+   for ( int idx = threadIdx.x; idx < chunk * 16; idx += 256 ) {
       const int r = idx / chunk, j = idx % chunk;
-      xs[ j * 16 + r ] = x[ (long)r * ldx + c0 + j ];
+      xs[ j * 16 + r ] = r < nrhs ? x[ (long)r * ldx + c0 + j ] : 0.0;
    }
    __syncthreads();
-   const int row = blockIdx.x * 256 + threadIdx.x;
-   if ( row >= npad ) return;
-   double acc[ 16 ];
+   const int row0 = blockIdx.x * 1024 + threadIdx.x;
+   double acc[ 4 ][ 16 ];
 #pragma unroll
-   for ( int r = 0; r < 16; ++r ) acc[ r ] = 0.0;
-   const double* a = A + (long)c0 * lda + row;
+   for ( int m = 0; m < 4; ++m )
+#pragma unroll
+      for ( int r = 0; r < 16; ++r ) acc[ m ][ r ] = 0.0;
+   bool ok[ 4 ];
+#pragma unroll
+   for ( int m = 0; m < 4; ++m ) ok[ m ] = row0 + 256 * m < npad;
+   const double* a = A + (long)c0 * lda + row0;
    for ( int j = 0; j < chunk; ++j ) {
-      const double v = a[ (long)j * lda ];
+      double v[ 4 ];
 #pragma unroll
-      for ( int r = 0; r < 16; ++r )
-         if ( r < nrhs ) acc[ r ] = acc[ r ] + v * xs[ j * 16 + r ];
+      for ( int m = 0; m < 4; ++m ) v[ m ] = ok[ m ] ? a[ (long)j * lda + 256 * m ] : 0.0;
+      const double* xr = xs + j * 16;
+#pragma unroll
+      for ( int r = 0; r < 16; ++r ) {
+         const double xv = xr[ r ];
+#pragma unroll
+         for ( int m = 0; m < 4; ++m ) acc[ m ][ r ] = acc[ m ][ r ] + v[ m ] * xv;
+      }
    }
    double* out = part + (long)blockIdx.y * 16 * npad;
-   for ( int r = 0; r < nrhs; ++r ) out[ (long)r * npad + row ] = acc[ r ];
+#pragma unroll
+   for ( int m = 0; m < 4; ++m )
+#pragma unroll
+      for ( int r = 0; r < 16; ++r )
+         if ( ok[ m ] && r < nrhs ) out[ (long)r * npad + row0 + 256 * m ] = acc[ m ][ r ];
 }
 
 __device__ inline void atomic_max_nonneg( double* addr, double v ) {
@@ -500,7 +581,7 @@ __global__ void __launch_bounds__( 256 ) k_ir_update( int n3, int nrhs, double* 
    const int row = blockIdx.x * 256 + threadIdx.x;
    if ( row >= n3 ) return;
    for ( int k = 0; k < nrhs; ++k ) {
-      const double dv = (double)d[ (long)k * ldd + row ];
+      const double dv = (double)d[ (long)row * 16 + k ];
       const double xv = x[ (long)k * ldx + row ] + dv;
       x[ (long)k * ldx + row ] = xv;
       atomic_max_nonneg( &norms[ k ], fabs( dv ) );
@@ -519,50 +600,89 @@ __global__ void k_f2d( long n, const float* __restrict__ a, double* __restrict__
 // ------------------------------------------------------------------ host side
 
 template <typename T>
+void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta,
+                          T* C, long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb,
+                          BatchStride sc ) {
+   if ( M <= 0 || N <= 0 || batch <= 0 ) return;
+   dim3 grid( M / 128, N / 128, batch );
+   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32, 2.0 * M * N * K * batch, st );
+   if constexpr ( sizeof( T ) == 8 )
+      hipLaunchKernelGGL( ( k_gemm<double, 16> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
+                          zdiv, sa, sb, sc );
+   else
+      hipLaunchKernelGGL( ( k_gemm<float, 32> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
+                          zdiv, sa, sb, sc );
+}
+
+template <typename T>
 void launch_gemm( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta, T* C,
                   long ldc, int* check ) {
-   if ( M <= 0 || N <= 0 ) return;
-   dim3 grid( M / 128, N / 128 );
-   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32, 2.0 * M * N * K );
-   if constexpr ( sizeof( T ) == 8 )
-      hipLaunchKernelGGL( ( k_gemm<double, 16> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check );
-   else
-      hipLaunchKernelGGL( ( k_gemm<float, 32> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check );
+   launch_gemm_batched<T>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, 1, 1, {}, {}, {} );
 }
 template void launch_gemm<double>( hipStream_t, int, int, int, double, const double*, long, const double*, long, double,
                                    double*, long, int* );
 template void launch_gemm<float>( hipStream_t, int, int, int, float, const float*, long, const float*, long, float, float*,
                                   long, int* );
 
-// The blocked factorization without exchanges (the fast path).  check is set when partial
-// pivoting would have exchanged rows somewhere; the factors are then not LAPACK's and the
-// caller must rerun getrf_pivoted on the original matrix.
+// The blocked factorization without exchanges (the fast path), with a look-ahead of one block
+// column: the trailing update of step k is split into the next block column (on st, where the
+// diagonal block and the panels of step k+1 follow at once) and the rest (on st2), so the
+// one-workgroup diagonal factorization of step k+1 runs while the GPU is busy with the bulk of
+// step k's update.  ev_u and ev_g order the two streams (U panel k before its update on st2; the
+// update on st2 before U panel k+1, which reads the rows it writes).  check is set when partial
+// pivoting would have exchanged rows somewhere; the caller must then rerun lu_pivoted on the
+// original matrix.
 template <typename T>
-void lu_nopiv( hipStream_t st, int npad, T* A, long lda, T* inv, int* check ) {
+void lu_nopiv( hipStream_t st, hipStream_t st2, hipEvent_t ev_u, hipEvent_t ev_g, int npad, T* A, long lda, T* inv,
+               int* check ) {
    const int nb = kNB, nblk = npad / nb;
-   for ( int k = 0; k < nblk; ++k ) {
+   auto diag = [&]( int k ) {
       const int kk = k * nb;
+      ProfScope prof( kKDiag, 2.0 * nb * nb * nb, st );  // LU 2/3 nb^3 + two triangular inverses 2/3 nb^3
+      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, A + (long)kk * lda + kk, lda,
+                          inv + (long)k * 2 * nb * nb, check );
+   };
+   auto lpanel = [&]( int k ) {  // A_below <- A_below U^-1 (checks |l| <= 1)
+      const int kk = k * nb, rest = npad - kk - nb;
       T* Akk = A + (long)kk * lda + kk;
-      T* invk = inv + (long)k * 2 * nb * nb;
-      {
-         ProfScope prof( kKDiag, 2.0 * nb * nb * nb );  // LU 2/3 nb^3 + two triangular inverses 2/3 nb^3
-         hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, Akk, lda, invk, check );
+      launch_gemm<T>( st, rest, nb, nb, T( 1 ), Akk + nb, lda, inv + (long)k * 2 * nb * nb + nb * nb, nb, T( 0 ), Akk + nb, lda,
+                      check );
+   };
+   auto upanel = [&]( int k ) {  // A_right <- L^-1 A_right
+      const int kk = k * nb, rest = npad - kk - nb;
+      T* Akk = A + (long)kk * lda + kk;
+      launch_gemm<T>( st, nb, rest, nb, T( 1 ), inv + (long)k * 2 * nb * nb, nb, Akk + (long)nb * lda, lda, T( 0 ),
+                      Akk + (long)nb * lda, lda, nullptr );
+   };
+   diag( 0 );
+   if ( nblk == 1 ) return;
+   lpanel( 0 );
+   upanel( 0 );
+   for ( int k = 0; k + 1 < nblk; ++k ) {
+      const int kk = k * nb, rest = npad - kk - nb;
+      T* Akk = A + (long)kk * lda + kk;
+      T* L = Akk + nb;
+      T* U = Akk + (long)nb * lda;
+      // next block column of the trailing matrix, on st
+      launch_gemm<T>( st, rest, nb, nb, T( -1 ), L, lda, U, lda, T( 1 ), U + nb, lda, nullptr );
+      // the rest of the trailing update, on st2, after U panel k
+      if ( rest > nb ) {
+         BH_HIP( hipEventRecord( ev_u, st ) );
+         BH_HIP( hipStreamWaitEvent( st2, ev_u, 0 ) );
+         launch_gemm<T>( st2, rest, rest - nb, nb, T( -1 ), L, lda, U + (long)nb * lda, lda, T( 1 ), U + (long)nb * lda + nb,
+                         lda, nullptr );
+         BH_HIP( hipEventRecord( ev_g, st2 ) );
       }
-      const int rest = npad - kk - nb;
-      if ( rest <= 0 ) break;
-      T* Abelow = Akk + nb;                 // rows kk+nb.., columns kk..kk+nb
-      T* Aright = Akk + (long)nb * lda;     // rows kk..kk+nb, columns kk+nb..
-      T* Atrail = Aright + nb;
-      // L panel: A_below <- A_below U^-1 (checks |l| <= 1)
-      launch_gemm<T>( st, rest, nb, nb, T( 1 ), Abelow, lda, invk + nb * nb, nb, T( 0 ), Abelow, lda, check );
-      // U panel: A_right <- L^-1 A_right
-      launch_gemm<T>( st, nb, rest, nb, T( 1 ), invk, nb, Aright, lda, T( 0 ), Aright, lda, nullptr );
-      // trailing update
-      launch_gemm<T>( st, rest, rest, nb, T( -1 ), Abelow, lda, Aright, lda, T( 1 ), Atrail, lda, nullptr );
+      diag( k + 1 );
+      if ( k + 2 < nblk ) {
+         lpanel( k + 1 );
+         BH_HIP( hipStreamWaitEvent( st, ev_g, 0 ) );
+         upanel( k + 1 );
+      }
    }
 }
-template void lu_nopiv<double>( hipStream_t, int, double*, long, double*, int* );
-template void lu_nopiv<float>( hipStream_t, int, float*, long, float*, int* );
+template void lu_nopiv<double>( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, double*, long, double*, int* );
+template void lu_nopiv<float>( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, float*, long, float*, int* );
 
 // True partial pivoting (slow path): the panel is factored column by column with a grid-wide
 // pivot search; interchanges are applied across the whole row at once.
@@ -592,28 +712,91 @@ void lu_pivoted( hipStream_t st, int npad, T* A, long lda, T* inv, int* ipiv, in
 template void lu_pivoted<double>( hipStream_t, int, double*, long, double*, int*, int*, double*, int* );
 template void lu_pivoted<float>( hipStream_t, int, float*, long, float*, int*, int*, double*, int* );
 
-// Solves with the factors: b (npad x nrhs) in place.  pivoted: apply ipiv first.  sync holds
-// nblk + 2 ints of scratch (flags, ticket, error); the error flag is left at sync[nblk + 1].
+constexpr int kSkinnySlice = 256;  // columns of K per workgroup of k_skinny_part
+
+// Y = beta Y + alpha A X (A column-major M x K, X and Y row-interleaved); P is scratch of
+// (K / kSkinnySlice) M 16 elements.  M must be a multiple of 64, K of kSkinnySlice.
 template <typename T>
-void lu_solve( hipStream_t st, int npad, int n3, const T* LU, long lda, const T* inv, const int* ipiv, bool pivoted,
-               int nrhs, T* b, long ldb, int* sync ) {
-   const int nblk = npad / kNB;
-   const int spin_limit = 1 << 24;
-   if ( pivoted ) hipLaunchKernelGGL( k_apply_ipiv<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, nrhs, ipiv, b, ldb );
-   int* done = sync;
-   int* ticket = sync + nblk;
-   int* err = sync + nblk + 1;
-   for ( int lower = 1; lower >= 0; --lower ) {
-      hipLaunchKernelGGL( k_zero_ints, dim3( 1 ), dim3( 256 ), 0, st, nblk + 1, sync );
-      ProfScope prof( kKTrsm, 1.0 * npad * npad * nrhs );  // half of the 2 n^2 nrhs of both triangles
-      hipLaunchKernelGGL( k_trsm_chain<T>, dim3( nblk ), dim3( 256 ), 0, st, nblk, lower, nrhs, LU, lda, inv, b, ldb, ticket,
-                          done, err, spin_limit );
+void launch_skinny( hipStream_t st, int M, int K, T alpha, const T* A, long lda, const T* X, T beta, T* Y, T* P ) {
+   if ( M <= 0 ) return;
+   const int nkc = ( K + kSkinnySlice - 1 ) / kSkinnySlice;
+   ProfScope prof( kKTrsm, 2.0 * M * K * 11, st );
+   hipLaunchKernelGGL( k_skinny_part<T>, dim3( M / 64, nkc ), dim3( 256 ), 0, st, M, K, kSkinnySlice, A, lda, X, P );
+   const long n16 = (long)M * 16;
+   hipLaunchKernelGGL( k_skinny_reduce<T>, dim3( (unsigned)std::min<long>( 2048, ( n16 + 255 ) / 256 ) ), dim3( 256 ), 0, st,
+                       n16, nkc, alpha, P, beta, Y );
+}
+
+// Inverses of the S x S diagonal super-blocks of L (unit lower) and U, from the 128 x 128 block
+// inverses by recursive doubling: with A, B the inverses of two adjacent s x s diagonal blocks and
+// C the block between them, inv([A 0; C B]) has -B^-1 C A^-1 below (and inv of upper: -A^-1 C B^-1
+// right of the diagonal).  Every level is two batched GEMMs per triangle over all pairs of all
+// super-blocks.  tmp holds npad * S / 2 elements.
+template <typename T>
+void build_super_inverses( hipStream_t st, int npad, int S, const T* LU, long lda, const T* inv, T* XL, T* XU, T* tmp ) {
+   const int nsb = npad / S;
+   hipLaunchKernelGGL( k_super_init<T>, dim3( 1024 ), dim3( 256 ), 0, st, S, nsb, inv, XL, XU );
+   const long SS = (long)S * S;
+   for ( int s = kNB; s < S; s *= 2 ) {
+      const int zdiv = S / ( 2 * s ), batch = nsb * zdiv;
+      const long ss = (long)s * s;
+      BatchStride t_str { zdiv * ss, ss };
+      // L: T = C A^-1, X21 = -B^-1 T
+      BatchStride c_l { (long)S + (long)S * lda, 2L * s + 2L * s * lda };
+      BatchStride a_x { SS, 2L * s * ( S + 1 ) };
+      launch_gemm_batched<T>( st, s, s, s, T( 1 ), LU + s, lda, XL, S, T( 0 ), tmp, s, nullptr, batch, zdiv, c_l, a_x, t_str );
+      BatchStride x21 { SS, 2L * s * ( S + 1 ) };
+      launch_gemm_batched<T>( st, s, s, s, T( -1 ), XL + (long)s * ( S + 1 ), S, tmp, s, T( 0 ), XL + s, S, nullptr, batch, zdiv,
+                              a_x, t_str, x21 );
+      // U: T = C B^-1, X12 = -A^-1 T
+      BatchStride c_u { (long)S + (long)S * lda, 2L * s + 2L * s * lda };
+      launch_gemm_batched<T>( st, s, s, s, T( 1 ), LU + (long)s * lda, lda, XU + (long)s * ( S + 1 ), S, T( 0 ), tmp, s, nullptr,
+                              batch, zdiv, c_u, a_x, t_str );
+      launch_gemm_batched<T>( st, s, s, s, T( -1 ), XU, S, tmp, s, T( 0 ), XU + (long)s * S, S, nullptr, batch, zdiv, a_x, t_str,
+                              x21 );
    }
 }
-template void lu_solve<double>( hipStream_t, int, int, const double*, long, const double*, const int*, bool, int, double*,
-                                long, int* );
-template void lu_solve<float>( hipStream_t, int, int, const float*, long, const float*, const int*, bool, int, float*,
-                               long, int* );
+template void build_super_inverses<double>( hipStream_t, int, int, const double*, long, const double*, double*, double*,
+                                            double* );
+template void build_super_inverses<float>( hipStream_t, int, int, const float*, long, const float*, float*, float*, float* );
+
+// Solves L U x = P b with the factors and the super-block inverses.  b, y, x are row-interleaved
+// npad x 16 blocks: b is consumed (row-exchanged when pivoted, then overwritten by the forward
+// updates), y is work space, the solution goes to x.  Two launches per super-block and triangle:
+// the inverse times the block, then the update of the rest of the right-hand side.
+template <typename T>
+void lu_solve( hipStream_t st, int npad, int n3, int S, const T* LU, long lda, const T* XL, const T* XU, const int* ipiv,
+               bool pivoted, T* b, T* y, T* x, T* P ) {
+   const int nsb = npad / S;
+   if ( pivoted ) hipLaunchKernelGGL( k_apply_ipiv_il<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, ipiv, b );
+   for ( int q = 0; q < nsb; ++q ) {
+      const long q0 = (long)q * S;
+      launch_skinny<T>( st, S, S, T( 1 ), XL + q * (long)S * S, S, b + q0 * 16, T( 0 ), y + q0 * 16, P );
+      launch_skinny<T>( st, npad - (int)q0 - S, S, T( -1 ), LU + q0 * lda + q0 + S, lda, y + q0 * 16, T( 1 ),
+                        b + ( q0 + S ) * 16, P );
+   }
+   for ( int q = nsb - 1; q >= 0; --q ) {
+      const long q0 = (long)q * S;
+      launch_skinny<T>( st, S, S, T( 1 ), XU + q * (long)S * S, S, y + q0 * 16, T( 0 ), x + q0 * 16, P );
+      launch_skinny<T>( st, (int)q0, S, T( -1 ), LU + q0 * lda, lda, x + q0 * 16, T( 1 ), y, P );
+   }
+}
+template void lu_solve<double>( hipStream_t, int, int, int, const double*, long, const double*, const double*, const int*,
+                                bool, double*, double*, double*, double* );
+template void lu_solve<float>( hipStream_t, int, int, int, const float*, long, const float*, const float*, const int*, bool,
+                               float*, float*, float*, float* );
+
+template <typename S, typename D>
+void to_interleaved( hipStream_t st, int npad, int nrhs, const S* b, long ldb, D* v ) {
+   hipLaunchKernelGGL( ( k_to_interleaved<S, D> ), dim3( 1024 ), dim3( 256 ), 0, st, npad, nrhs, b, ldb, v );
+}
+template <typename S, typename D>
+void from_interleaved( hipStream_t st, int n, int nrhs, const S* v, D* b, long ldb ) {
+   hipLaunchKernelGGL( ( k_from_interleaved<S, D> ), dim3( 1024 ), dim3( 256 ), 0, st, n, nrhs, v, b, ldb );
+}
+template void to_interleaved<double, double>( hipStream_t, int, int, const double*, long, double* );
+template void to_interleaved<double, float>( hipStream_t, int, int, const double*, long, float* );
+template void from_interleaved<double, double>( hipStream_t, int, int, const double*, double*, long );
 
 // Column chunk of the residual product: the largest of 512, 256, 128 that divides npad.
 static int resid_chunk( int npad ) {
@@ -627,8 +810,8 @@ void residual( hipStream_t st, int npad, int n3, int nrhs, const double* A, long
                const double* b, long ldb, double* r, long ldr, double* part, double* norms ) {
    const int chunk = resid_chunk( npad );
    const int nchunks = npad / chunk;
-   dim3 grid( ( npad + 255 ) / 256, nchunks );
-   ProfScope prof( kKResid, 2.0 * npad * npad * nrhs );
+   dim3 grid( ( npad + 1023 ) / 1024, nchunks );
+   ProfScope prof( kKResid, 2.0 * npad * npad * nrhs, st );
    hipLaunchKernelGGL( k_resid_part, grid, dim3( 256 ), chunk * 16 * sizeof( double ), st, npad, chunk, nrhs, A, lda, x,
                        ldx, part );
    hipLaunchKernelGGL( k_resid_final, dim3( ( npad + 255 ) / 256 ), dim3( 256 ), 0, st, npad, n3, nchunks, nrhs, part, b,

@@ -36,11 +36,17 @@ void launch_pad_identity( hipStream_t, int, int, double*, long );
 void launch_entries( hipStream_t, int, double, const double*, const double*, const double*, int, int, int, int, double*,
                      long );
 template <typename T>
-void lu_nopiv( hipStream_t, int, T*, long, T*, int* );
+void lu_nopiv( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, T*, long, T*, int* );
 template <typename T>
 void lu_pivoted( hipStream_t, int, T*, long, T*, int*, int*, double*, int* );
 template <typename T>
-void lu_solve( hipStream_t, int, int, const T*, long, const T*, const int*, bool, int, T*, long, int* );
+void build_super_inverses( hipStream_t, int, int, const T*, long, const T*, T*, T*, T* );
+template <typename T>
+void lu_solve( hipStream_t, int, int, int, const T*, long, const T*, const T*, const int*, bool, T*, T*, T*, T* );
+template <typename S, typename D>
+void to_interleaved( hipStream_t, int, int, const S*, long, D* );
+template <typename S, typename D>
+void from_interleaved( hipStream_t, int, int, const S*, D*, long );
 void residual( hipStream_t, int, int, int, const double*, long, const double*, long, const double*, long, double*, long,
                double*, double* );
 size_t residual_part_elems( int );
@@ -72,6 +78,9 @@ struct DevBuf {
 };
 
 constexpr int kMaxRhs = 16;
+// Order of the diagonal super-blocks whose inverses the triangular solves use; the matrix is
+// padded to a multiple of it.
+constexpr int kSuper = 1024;
 constexpr int kPairStrip = 64;  // source triangles per workgroup of the pair kernel
 
 struct PhaseTimer {
@@ -101,6 +110,8 @@ struct PhaseTimer {
 struct bhg_solver {
    int device = 0;
    hipStream_t st = nullptr;
+   hipStream_t st2 = nullptr;          // look-ahead stream of the factorization
+   hipEvent_t ev_u = nullptr, ev_g = nullptr;
    std::string last_error;
    std::deque<std::string> log;
    // mesh
@@ -111,22 +122,22 @@ struct bhg_solver {
    bhg::DevBuf<double> raw_v, v, area, xc, selfb, xmu, src, gdef, scal;
    bhg::DevBuf<int> comp;
    // matrix and factors
-   bhg::DevBuf<double> a64, inv64;
-   bhg::DevBuf<float> a32, inv32;
-   bhg::DevBuf<int> ipiv, flags, tsync;
+   bhg::DevBuf<double> a64, inv64, xl64, xu64, tmp64;
+   bhg::DevBuf<float> a32, inv32, xl32, xu32, tmp32;
+   bhg::DevBuf<int> ipiv, flags;
    bhg::DevBuf<double> pval;
    bhg::DevBuf<int> pidx;
    int factored = 0;          // 0, 64 or 32
    bool pivoted = false;      // the slow path ran (rows were exchanged)
    bool deflated = true;      // the factored matrix carries the deflation
    // vectors
-   bhg::DevBuf<double> b, x, r, part, norms, sums;
-   bhg::DevBuf<float> d32;
+   bhg::DevBuf<double> b, x, r, y64, x64il, p64, part, norms, sums;
+   bhg::DevBuf<float> d32, y32, x32, p32;
    std::vector<double> host_sums;
    bhg::PhaseTimer timer;
    bhg::LaunchProfiler prof;
    bhg_stats stats {};
-   double ir_tol = 2e-15;
+   double ir_tol = 1e-12;
    int ir_maxit = 30;
 
    void say( const char* fmt, ... ) __attribute__( ( format( printf, 2, 3 ) ) ) {
@@ -151,7 +162,7 @@ void set_device( bhg_solver* s ) { BH_HIP( hipSetDevice( s->device ) ); }
 void size_mesh( bhg_solver* s, int n ) {
    s->n = n;
    s->n3 = 3 * n;
-   s->npad = round_up( s->n3, kNB );
+   s->npad = round_up( s->n3, kSuper );
    s->v.ensure( 9 * (size_t)n );
    s->area.ensure( n );
    s->xc.ensure( 3 * (size_t)n );
@@ -208,7 +219,7 @@ void factor( bhg_solver* s, int precision ) {
    s->pivoted = false;
    if ( precision == 64 ) {
       s->inv64.ensure( (size_t)nblk * 2 * kNB * kNB );
-      lu_nopiv<double>( s->st, s->npad, s->a64.p, (long)np, s->inv64.p, s->flags.p );
+      lu_nopiv<double>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a64.p, (long)np, s->inv64.p, s->flags.p );
       if ( read_flag( s ) ) {
          s->say( "LU: a multiplier exceeded 1; refactoring with row exchanges" );
          assemble( s, s->deflated );
@@ -216,11 +227,15 @@ void factor( bhg_solver* s, int precision ) {
          lu_pivoted<double>( s->st, s->npad, s->a64.p, (long)np, s->inv64.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
          s->pivoted = true;
       }
+      s->xl64.ensure( np * kSuper );
+      s->xu64.ensure( np * kSuper );
+      s->tmp64.ensure( np * kSuper / 2 );
+      build_super_inverses<double>( s->st, s->npad, kSuper, s->a64.p, (long)np, s->inv64.p, s->xl64.p, s->xu64.p, s->tmp64.p );
    } else {
       s->a32.ensure( np * np );
       s->inv32.ensure( (size_t)nblk * 2 * kNB * kNB );
       convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
-      lu_nopiv<float>( s->st, s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
+      lu_nopiv<float>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
       if ( read_flag( s ) ) {
          s->say( "LU (FP32): a multiplier exceeded 1; refactoring with row exchanges" );
          convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
@@ -228,6 +243,10 @@ void factor( bhg_solver* s, int precision ) {
          lu_pivoted<float>( s->st, s->npad, s->a32.p, (long)np, s->inv32.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
          s->pivoted = true;
       }
+      s->xl32.ensure( np * kSuper );
+      s->xu32.ensure( np * kSuper );
+      s->tmp32.ensure( np * kSuper / 2 );
+      build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, (long)np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
    }
    s->factored = precision;
 }
@@ -239,23 +258,21 @@ void size_vectors( bhg_solver* s ) {
    s->b.ensure( need );
    s->x.ensure( need );
    s->r.ensure( need );
+   s->y64.ensure( need );
+   s->x64il.ensure( need );
+   s->p64.ensure( need * 4 );
    s->d32.ensure( need );
+   s->y32.ensure( need );
+   s->x32.ensure( need );
+   s->p32.ensure( need * 4 );
    s->part.ensure( residual_part_elems( s->npad ) );
    s->norms.ensure( 32 );
    s->sums.ensure( (size_t)kSumGroups * kSumSlots );
-   s->tsync.ensure( s->npad / kNB + 2 );
+
    if ( grow ) {
       BH_HIP( hipMemsetAsync( s->b.p, 0, need * sizeof( double ), s->st ) );
       BH_HIP( hipMemsetAsync( s->x.p, 0, need * sizeof( double ), s->st ) );
    }
-}
-
-// Waits for the stream and reports a triangular solve whose dependency wait ran out.
-void check_solve( bhg_solver* s ) {
-   int err = 0;
-   BH_HIP( hipMemcpyAsync( &err, s->tsync.p + s->npad / kNB + 1, sizeof err, hipMemcpyDeviceToHost, s->st ) );
-   BH_HIP( hipStreamSynchronize( s->st ) );
-   if ( err ) throw std::runtime_error( "triangular solve: a block dependency did not arrive in time" );
 }
 
 // Solves for the nrhs columns of s->b (npad x nrhs, rows >= n3 zero) into s->x.
@@ -264,10 +281,10 @@ void solve_device( bhg_solver* s, int nrhs ) {
    s->stats.ir_iterations = 0;
    s->stats.ir_correction = 0.0;
    if ( s->factored == 64 ) {
-      BH_HIP( hipMemcpyAsync( s->x.p, s->b.p, sizeof( double ) * np * nrhs, hipMemcpyDeviceToDevice, s->st ) );
-      lu_solve<double>( s->st, s->npad, s->n3, s->a64.p, np, s->inv64.p, s->ipiv.p, s->pivoted, nrhs, s->x.p, np,
-                        s->tsync.p );
-      check_solve( s );
+      to_interleaved<double, double>( s->st, s->npad, nrhs, s->b.p, np, s->r.p );
+      lu_solve<double>( s->st, s->npad, s->n3, kSuper, s->a64.p, np, s->xl64.p, s->xu64.p, s->ipiv.p, s->pivoted, s->r.p,
+                        s->y64.p, s->x64il.p, s->p64.p );
+      from_interleaved<double, double>( s->st, s->n3, nrhs, s->x64il.p, s->x.p, np );
       return;
    }
    if ( s->factored != 32 ) throw std::runtime_error( "solve before the factorization" );
@@ -279,17 +296,17 @@ void solve_device( bhg_solver* s, int nrhs ) {
       BH_HIP( hipMemsetAsync( s->norms.p, 0, 32 * sizeof( double ), s->st ) );
       if ( it == 0 ) {
          BH_HIP( hipMemsetAsync( s->x.p, 0, sizeof( double ) * np * nrhs, s->st ) );
-         convert_d2f( s->st, np * nrhs, s->b.p, s->d32.p );
+         to_interleaved<double, float>( s->st, s->npad, nrhs, s->b.p, np, s->d32.p );
       } else {
          residual( s->st, s->npad, s->n3, nrhs, s->a64.p, np, s->x.p, np, s->b.p, np, s->r.p, np, s->part.p, s->norms.p );
-         convert_d2f( s->st, np * nrhs, s->r.p, s->d32.p );
+         to_interleaved<double, float>( s->st, s->npad, nrhs, s->r.p, np, s->d32.p );
          BH_HIP( hipMemsetAsync( s->norms.p, 0, 32 * sizeof( double ), s->st ) );
       }
-      lu_solve<float>( s->st, s->npad, s->n3, s->a32.p, np, s->inv32.p, s->ipiv.p, s->pivoted, nrhs, s->d32.p, np,
-                       s->tsync.p );
-      ir_update( s->st, s->n3, nrhs, s->x.p, np, s->d32.p, np, s->norms.p );
+      lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, s->pivoted, s->d32.p,
+                       s->y32.p, s->x32.p, s->p32.p );
+      ir_update( s->st, s->n3, nrhs, s->x.p, np, s->x32.p, np, s->norms.p );
       BH_HIP( hipMemcpyAsync( hn, s->norms.p, sizeof hn, hipMemcpyDeviceToHost, s->st ) );
-      check_solve( s );
+      BH_HIP( hipStreamSynchronize( s->st ) );
       double worst = 0.0;
       for ( int k = 0; k < nrhs; ++k ) {
          const double rel = hn[ 16 + k ] > 0 ? hn[ k ] / hn[ 16 + k ] : 0.0;
@@ -411,6 +428,9 @@ bhg_solver* bhg_create( int device ) {
    try {
       BH_HIP( hipSetDevice( device ) );
       BH_HIP( hipStreamCreateWithFlags( &s->st, hipStreamNonBlocking ) );
+      BH_HIP( hipStreamCreateWithFlags( &s->st2, hipStreamNonBlocking ) );
+      BH_HIP( hipEventCreateWithFlags( &s->ev_u, hipEventDisableTiming ) );
+      BH_HIP( hipEventCreateWithFlags( &s->ev_g, hipEventDisableTiming ) );
       s->timer.init();
       s->flags.ensure( 4 );
    } catch ( ... ) {
@@ -423,9 +443,14 @@ void bhg_destroy( bhg_solver* s ) {
    if ( !s ) return;
    (void)hipSetDevice( s->device );
    if ( s->st ) (void)hipStreamSynchronize( s->st );
-   hipStream_t st = s->st;
+   if ( s->st2 ) (void)hipStreamSynchronize( s->st2 );
+   hipStream_t st = s->st, st2 = s->st2;
+   hipEvent_t e1 = s->ev_u, e2 = s->ev_g;
    delete s;
+   if ( e1 ) (void)hipEventDestroy( e1 );
+   if ( e2 ) (void)hipEventDestroy( e2 );
    if ( st ) (void)hipStreamDestroy( st );
+   if ( st2 ) (void)hipStreamDestroy( st2 );
 }
 
 const char* bhg_last_error( bhg_solver* s ) { return s ? s->last_error.c_str() : "no solver"; }
