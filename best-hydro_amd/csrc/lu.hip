@@ -46,12 +46,15 @@ __device__ inline T tabs( T v ) {
 }  // namespace
 
 // C (M x N) = alpha A (M x K) B (K x N) + beta C, column-major; M, N multiples of 128, K of BK.
-// 256 threads: four waves in a 2 x 2 arrangement, each owning a 64 x 64 block of the tile as
-// 4 x 4 MFMA 16 x 16 accumulators.  A and B slices go through LDS ([k][m] and [k][n], so a
-// wave's 16 lanes of one k read 16 consecutive elements); the next slice is fetched into
-// registers while the current one is multiplied.  C may alias A (when N == 128: each workgroup
-// reads its whole row strip before it writes) or B (when M == 128).  check != nullptr: set
-// *check when any |C| > 1 (the pivot test of the panel below the diagonal block).
+// 256 threads: four waves in a 2 x 2 arrangement, each owning a 64 x 64 block of the 128 x 128
+// tile as 4 x 4 MFMA 16 x 16 accumulators.  A and B slices of BK columns pass through LDS in two
+// stages: while one stage feeds the MFMAs, the next slice is already in flight from global memory
+// (one barrier per slice).  In LDS each wave's 64 rows (columns for B) are stored permuted, row
+// 16 t + i at slot 4 i + t, so the four fragments a lane needs for one k come as one 16-byte
+// (FP32) or two 16-byte (FP64) reads.  With beta != 0 the C tile is loaded into registers before
+// the main loop.  C may alias A (when N == 128: each workgroup reads its whole row strip before
+// it writes) or B (when M == 128).  check != nullptr: set *check when any |C| > 1 (the pivot test
+// of the panel below the diagonal block).
 // Batched over blockIdx.z: batch z = zdiv * q + t offsets A by q sa.hi + t sa.lo, and likewise B, C.
 struct BatchStride {
    long hi = 0, lo = 0;
@@ -61,16 +64,17 @@ template <typename T, int BK>
 __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
                                                 T beta, T* C, long ldc, int* check, int zdiv, BatchStride sa,
                                                 BatchStride sb, BatchStride sc ) {
-   constexpr int BM = 128, BN = 128, PAD = 4;
+   constexpr int BM = 128, BN = 128, PAD = 8;
+   constexpr int LDW = BM + PAD;  // LDS row length (elements)
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T As[ 2 ][ BK ][ LDW ];
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T Bs[ 2 ][ BK ][ LDW ];
+   using v4 = typename Mfma<T>::v4;
    {
       const long q = blockIdx.z / zdiv, t = blockIdx.z % zdiv;
       A += q * sa.hi + t * sa.lo;
       B += q * sb.hi + t * sb.lo;
       C += q * sc.hi + t * sc.lo;
    }
-   __shared__ T As[ BK ][ BM + PAD ];
-   __shared__ T Bs[ BK ][ BN + PAD ];
-   using v4 = typename Mfma<T>::v4;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave & 1, wn = wave >> 1;
    const int bm = blockIdx.x, bn = blockIdx.y;
@@ -82,12 +86,32 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    constexpr int BTPN = BK / BE;      // threads per column of B
    const int a_k = tid / ATPK, a_m = ( tid % ATPK ) * AE;
    const int b_n = tid / BTPN, b_k = ( tid % BTPN ) * BE;
+   // permuted slot of row (column) m of the tile: wave block m / 64, t = (m % 64) / 16, i = m % 16
+   auto slot = []( int m ) { return ( m & ~63 ) + ( ( m & 15 ) << 2 ) + ( ( m >> 4 ) & 3 ); };
+   const int b_slot = slot( b_n );
    T ra[ AE ], rb[ BE ];
    v4 acc[ 4 ][ 4 ];
 #pragma unroll
    for ( int t = 0; t < 4; ++t )
 #pragma unroll
       for ( int u = 0; u < 4; ++u ) acc[ t ][ u ] = v4{ 0, 0, 0, 0 };
+   // C tile of this lane, loaded ahead (element r of accumulator (t, u)).
+   const int li = lane & 15, lk = lane >> 4;
+   auto crow = [&]( int t, int r ) { return wm * 64 + t * 16 + ( sizeof( T ) == 8 ? lk + 4 * r : 4 * lk + r ); };
+   // FP64 keeps its 64 accumulators per lane in AGPRs only without the prefetched tile: there it
+   // reads C in the epilogue.
+   constexpr bool kPre = sizeof( T ) == 4;
+   T cpre[ 4 ][ 4 ][ 4 ];
+   if ( kPre && beta != T( 0 ) ) {
+#pragma unroll
+      for ( int t = 0; t < 4; ++t )
+#pragma unroll
+         for ( int u = 0; u < 4; ++u ) {
+            const T* c = C + (long)( bn * BN + wn * 64 + u * 16 + li ) * ldc + bm * BM;
+#pragma unroll
+            for ( int r = 0; r < 4; ++r ) cpre[ t ][ u ][ r ] = c[ crow( t, r ) ];
+         }
+   }
    const int nk = K / BK;
    auto gload = [&]( int kt ) {
       const T* pa = Ab + (long)( kt * BK + a_k ) * lda + a_m;
@@ -97,44 +121,45 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
 #pragma unroll
       for ( int e = 0; e < BE; ++e ) rb[ e ] = pb[ e ];
    };
+   auto sstore = [&]( int stg ) {
+#pragma unroll
+      for ( int e = 0; e < AE; ++e ) As[ stg ][ a_k ][ slot( a_m + e ) ] = ra[ e ];
+#pragma unroll
+      for ( int e = 0; e < BE; ++e ) Bs[ stg ][ b_k + e ][ b_slot ] = rb[ e ];
+   };
    gload( 0 );
+   sstore( 0 );
+   __syncthreads();
    for ( int kt = 0; kt < nk; ++kt ) {
-      __syncthreads();
-#pragma unroll
-      for ( int e = 0; e < AE; ++e ) As[ a_k ][ a_m + e ] = ra[ e ];
-#pragma unroll
-      for ( int e = 0; e < BE; ++e ) Bs[ b_k + e ][ b_n ] = rb[ e ];
-      __syncthreads();
+      const int stg = kt & 1;
       if ( kt + 1 < nk ) gload( kt + 1 );
 #pragma unroll
       for ( int kk = 0; kk < BK; kk += 4 ) {
+         const int kr = kk + lk;
+         const T* pa = &As[ stg ][ kr ][ wm * 64 + li * 4 ];
+         const T* pb = &Bs[ stg ][ kr ][ wn * 64 + li * 4 ];
          T af[ 4 ], bf[ 4 ];
-         const int kr = kk + ( lane >> 4 ), c16 = lane & 15;
 #pragma unroll
-         for ( int t = 0; t < 4; ++t ) af[ t ] = As[ kr ][ wm * 64 + t * 16 + c16 ];
-#pragma unroll
-         for ( int u = 0; u < 4; ++u ) bf[ u ] = Bs[ kr ][ wn * 64 + u * 16 + c16 ];
+         for ( int t = 0; t < 4; ++t ) af[ t ] = pa[ t ], bf[ t ] = pb[ t ];
 #pragma unroll
          for ( int t = 0; t < 4; ++t )
 #pragma unroll
             for ( int u = 0; u < 4; ++u ) acc[ t ][ u ] = Mfma<T>::run( af[ t ], bf[ u ], acc[ t ][ u ] );
       }
+      if ( kt + 1 < nk ) sstore( stg ^ 1 );
+      __syncthreads();
    }
    bool big = false;
 #pragma unroll
    for ( int t = 0; t < 4; ++t )
 #pragma unroll
       for ( int u = 0; u < 4; ++u ) {
-         const int col = bn * BN + wn * 64 + u * 16 + ( lane & 15 );
-         T* c = C + (long)col * ldc + bm * BM + wm * 64 + t * 16;
+         T* c = C + (long)( bn * BN + wn * 64 + u * 16 + li ) * ldc + bm * BM;
 #pragma unroll
          for ( int r = 0; r < 4; ++r ) {
-            // Accumulator element r of this lane: row 4 (lane / 16) + r for the FP32 instruction,
-            // row lane / 16 + 4 r for the FP64 one (measured lane maps of the two instructions).
-            const int row = sizeof( T ) == 8 ? ( lane >> 4 ) + 4 * r : 4 * ( lane >> 4 ) + r;
             T v = alpha * acc[ t ][ u ][ r ];
-            if ( beta != T( 0 ) ) v = v + beta * c[ row ];
-            c[ row ] = v;
+            if ( beta != T( 0 ) ) v = v + beta * ( kPre ? cpre[ t ][ u ][ r ] : c[ crow( t, r ) ] );
+            c[ crow( t, r ) ] = v;
             big |= tabs( v ) > T( 1 );
          }
       }
