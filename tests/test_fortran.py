@@ -1,0 +1,44 @@
+"""The Fortran host over iso_c_binding (best-hydro_amd/fortran): the best_hip module binds the
+handle API of include/best_hip.h, and bh_solve solves one BEST mesh file with it."""
+import os
+import subprocess
+
+import pytest
+from conftest import GOLDEN, ROOT, rel
+
+EXE = os.path.join(ROOT, "best-hydro_amd", "fortran", "bh_solve")
+
+needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="bh_solve not built (make -C best-hydro_amd/fortran)")
+
+
+@needs_exe
+def test_usage_without_arguments():
+    p = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2
+    assert "usage: bh_solve" in p.stdout
+
+
+def _parse(out):
+    vals = {}
+    for line in out.splitlines():
+        for key in ("DTRANS", "DR (1/s)", "CHI*V", "CHI "):
+            if line.startswith(key):
+                vals[key.split()[0]] = float(line.split()[-1])
+    return vals
+
+
+@needs_exe
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [32, 64])
+def test_fortran_solve_matches_oracle(oracle, prec):
+    path = os.path.join(GOLDEN, "2eu7_02562")
+    print(f"bh_solve {path} {prec}", flush=True)
+    p = subprocess.run([EXE, path, str(prec), "1", "1", "0"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    got = _parse(p.stdout)
+    v1, v2, v3, ta, tv = oracle.read_best(path)
+    ref = oracle.best_run(v1, v2, v3, ta, tv, iarea=1, cg=1.0, ivisco=1, icov=0)
+    assert rel(got["DTRANS"], ref["dtrans"]) < 1e-10
+    assert rel(got["DR"], ref["dr"]) < 1e-10
+    assert rel(got["CHI*V"], ref["chiprime"]) < 1e-10
+    assert rel(got["CHI"], ref["chi"]) < 1e-10
