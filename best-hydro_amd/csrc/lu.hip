@@ -166,159 +166,365 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    if ( check && big ) atomicOr( check, 1 );
 }
 
+// ------------------------------------------------------------------ the diagonal block
+
+constexpr int kLd = kNB + 1;  // leading dimension of the diagonal block in LDS
+
+// v (a wave-uniform lane's value) broadcast to the whole wave.
+template <typename T>
+__device__ inline T bcast( T v, int lane );
+template <>
+__device__ inline float bcast<float>( float v, int lane ) {
+   return __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), lane ) );
+}
+template <>
+__device__ inline double bcast<double>( double v, int lane ) {
+   const long long x = __double_as_longlong( v );
+   const unsigned lo = (unsigned)__builtin_amdgcn_readlane( (int)x, lane );
+   const unsigned hi = (unsigned)__builtin_amdgcn_readlane( (int)( x >> 32 ), lane );
+   return __longlong_as_double( (long long)( ( (unsigned long long)hi << 32 ) | lo ) );
+}
+
+// How a 32 x 32 tile operand is read: whole, as the unit lower triangle of its strict lower part,
+// or as its upper triangle (diagonal included).
+enum TileOp { kOpFull, kOpLowerUnit, kOpUpper };
+
+template <int OP, typename T>
+__device__ __forceinline__ T tile_elem( T v, int r, int c ) {
+   if ( OP == kOpLowerUnit ) return r > c ? v : ( r == c ? T( 1 ) : T( 0 ) );
+   if ( OP == kOpUpper ) return r <= c ? v : T( 0 );
+   return v;
+}
+
+// acc (a 32 x 32 tile as 2 x 2 MFMA 16 x 16 blocks, one wave) += sa op(A) op(B) over k = 0..31;
+// A and B are 32 x 32 tiles in LDS (leading dimension kLd); sa is +1 or -1.
+template <typename T, int OA, int OB>
+__device__ __forceinline__ void tile_mma( typename Mfma<T>::v4 ( &acc )[ 2 ][ 2 ], const T* A, const T* B, T sa, int lane ) {
+   const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+   for ( int kk = 0; kk < 8; ++kk ) {
+      const int k = 4 * kk + lk;
+      T a[ 2 ], b[ 2 ];
+#pragma unroll
+      for ( int h = 0; h < 2; ++h ) {
+         const int r = 16 * h + lr;
+         a[ h ] = sa * tile_elem<OA>( A[ k * kLd + r ], r, k );
+         b[ h ] = tile_elem<OB>( B[ r * kLd + k ], k, r );
+      }
+#pragma unroll
+      for ( int bi = 0; bi < 2; ++bi )
+#pragma unroll
+         for ( int bj = 0; bj < 2; ++bj ) acc[ bi ][ bj ] = Mfma<T>::run( a[ bi ], b[ bj ], acc[ bi ][ bj ] );
+   }
+}
+
+// Row of accumulator element r of an MFMA 16 x 16 block held by a lane (column: lane % 16).
+template <typename T>
+__device__ __forceinline__ int acc_row( int lane, int r ) {
+   return sizeof( T ) == 8 ? ( lane >> 4 ) + 4 * r : 4 * ( lane >> 4 ) + r;
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_zero( typename Mfma<T>::v4 ( &acc )[ 2 ][ 2 ] ) {
+#pragma unroll
+   for ( int bi = 0; bi < 2; ++bi )
+#pragma unroll
+      for ( int bj = 0; bj < 2; ++bj )
+#pragma unroll
+         for ( int r = 0; r < 4; ++r ) acc[ bi ][ bj ][ r ] = T( 0 );
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_load( typename Mfma<T>::v4 ( &acc )[ 2 ][ 2 ], const T* C, int lane ) {
+#pragma unroll
+   for ( int bi = 0; bi < 2; ++bi )
+#pragma unroll
+      for ( int bj = 0; bj < 2; ++bj )
+#pragma unroll
+         for ( int r = 0; r < 4; ++r )
+            acc[ bi ][ bj ][ r ] = C[ ( 16 * bj + ( lane & 15 ) ) * kLd + 16 * bi + acc_row<T>( lane, r ) ];
+}
+
+// Stores the tile; returns true if any |entry| > 1 (or is NaN) when check is set.
+template <typename T>
+__device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[ 2 ][ 2 ], T* C, int lane, bool check ) {
+   bool big = false;
+#pragma unroll
+   for ( int bi = 0; bi < 2; ++bi )
+#pragma unroll
+      for ( int bj = 0; bj < 2; ++bj )
+#pragma unroll
+         for ( int r = 0; r < 4; ++r ) {
+            const T v = acc[ bi ][ bj ][ r ];
+            if ( check ) big |= !( tabs( v ) <= T( 1 ) );
+            C[ ( 16 * bj + ( lane & 15 ) ) * kLd + 16 * bi + acc_row<T>( lane, r ) ] = v;
+         }
+   return big;
+}
+
+// One wave factors the 32 x 32 tile held as rows in registers (lane i and i + 32: row i, a[c] =
+// entry (i, c)) without exchanges: multipliers a_ij * (1 / a_jj), as DGETF2 scales; pivot-row
+// entries come from the pivot lane by readlane, so the 32 steps need no barrier.  Returns true
+// if some |a_ij| > |a_jj| at the step that eliminates column j (partial pivoting would exchange).
+template <typename T>
+__device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i ) {
+   bool big = false;
+#pragma unroll
+   for ( int j = 0; j < 32; ++j ) {
+      const T piv = bcast( a[ j ], j );
+      const T rp = T( 1 ) / piv;
+      const bool below = i > j;
+      big |= below && !( tabs( a[ j ] ) <= tabs( piv ) );
+      const T l = below ? a[ j ] * rp : T( 0 );
+#pragma unroll
+      for ( int c = j + 1; c < 32; ++c ) a[ c ] = a[ c ] - l * bcast( a[ c ], j );
+      a[ j ] = below ? l : a[ j ];
+   }
+   return big;
+}
+
+// L^-1 of the unit lower triangle in the strict lower part of the rows, in place (the upper part
+// is untouched): for k = 0..31, rows i > k get X(i, c) -= l_ik X(k, c) for c < k and X(i, k) = -l_ik.
+template <typename T>
+__device__ __forceinline__ void leaf_inv_lower( T ( &a )[ 32 ], int i ) {
+#pragma unroll
+   for ( int k = 0; k < 32; ++k ) {
+      const T li = i > k ? a[ k ] : T( 0 );
+#pragma unroll
+      for ( int c = 0; c < k; ++c ) a[ c ] = a[ c ] - li * bcast( a[ c ], k );
+      a[ k ] = i > k ? -li : a[ k ];
+   }
+}
+
+// U^-1 of the upper triangle (diagonal included) of the rows, in place (the strict lower part is
+// untouched), as U1^-1 D^-1 with U = D U1: each row is scaled by its own 1 / u_ii, the unit upper
+// triangle U1 is inverted by rows from the bottom (for k = 31..0, rows i < k get X(i, c) -= u1_ik
+// X(k, c) for c > k and X(i, k) = -u1_ik), and column c of the result is scaled by 1 / u_cc.
+template <typename T>
+__device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i ) {
+   T dg = a[ 0 ];
+#pragma unroll
+   for ( int c = 1; c < 32; ++c ) dg = i == c ? a[ c ] : dg;
+   const T rd = T( 1 ) / dg;
+#pragma unroll
+   for ( int c = 1; c < 32; ++c ) a[ c ] = c > i ? a[ c ] * rd : a[ c ];
+#pragma unroll
+   for ( int k = 31; k > 0; --k ) {
+      const T ui = i < k ? a[ k ] : T( 0 );
+#pragma unroll
+      for ( int c = k + 1; c < 32; ++c ) a[ c ] = a[ c ] - ui * bcast( a[ c ], k );
+      a[ k ] = i < k ? -ui : a[ k ];
+   }
+#pragma unroll
+   for ( int c = 0; c < 32; ++c ) {
+      const T x = i == c ? T( 1 ) : a[ c ];
+      a[ c ] = c >= i ? x * bcast( rd, c ) : a[ c ];
+   }
+}
+
 // The diagonal block at A (leading dimension lda): mode 0 factors it without exchanges (flagging
-// any |a_ij| > |a_jj| at the step that eliminates column j) and then inverts both triangles;
-// mode 1 only inverts an already-factored block.  inv receives L^-1 (unit lower) and U^-1
-// (upper) as two dense 128 x 128 column-major matrices.
+// any |a_ij| > |a_jj| at the step that eliminates column j, and any multiplier above 1 in
+// magnitude) and then inverts both triangles; mode 1 only inverts an already-factored block.
+// inv receives L^-1 (unit lower) and U^-1 (upper) as two dense 128 x 128 column-major matrices.
 //
-// One workgroup of 256 threads; thread (ty, tx) of a 16 x 16 grid keeps the 8 x 8 tile at rows
-// 8 ty.., columns 8 tx.. in registers.  The step loop runs over tile columns jb and, unrolled,
-// over the offset jo inside the tile, so every register index is a compile-time constant.  Each
-// step broadcasts pivot row and column through LDS (double-buffered: one barrier per step) and
-// applies the rank-1 update to the whole tile with multipliers and pivot-row entries set to zero
-// outside the active part, so the update itself has no per-element conditions.  The multipliers
-// are a_ij * (1 / a_jj), as DGETF2 scales.  The inversions follow in the same way: U^-1 by rows
-// from the bottom (X(j,:) = Y(j,:) / u_jj, then Y(i,:) -= u_ij X(j,:) for i < j, where a position
-// still holding u_ic becomes X's entry when column c is reached) and L^-1 by rows from the top
-// (unit diagonal).  The two touch disjoint halves of the block.
+// One workgroup of four waves with the block in LDS as a 4 x 4 grid of 32 x 32 tiles; blocked
+// right-looking LU with 32-column steps.  Step p: wave 0 factors the diagonal tile in registers
+// (leaf_lu), then wave 0 inverts its L and wave 1 its U in place (the two write disjoint halves
+// of the tile); the tiles below become A U_pp^-1 and those right L_pp^-1 A, and the trailing
+// tiles get A -= L U, all as 32 x 32 MFMA tile products spread over the waves.  The LU factors
+// go to A as they are final.  The inverses of the whole block follow from the tile inverses by
+// recursive doubling, inv([A 0; C B]) = [A^-1 0; -B^-1 C A^-1  B^-1] (and the upper analogue),
+// in place in LDS: 32 -> 64 (four independent tile tasks), then 64 -> 128.
 template <typename T>
 __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T* __restrict__ inv, int* check ) {
-   constexpr int NB = kNB, RB = 8;
-   __shared__ T colb[ 2 ][ NB ], rowb[ 2 ][ NB ], colL[ 2 ][ NB ], rowL[ 2 ][ NB ];
-   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-   const int r0 = ty * RB, c0 = tx * RB;
-   T a[ RB ][ RB ];
+   typedef typename Mfma<T>::v4 v4;
+   __shared__ T S[ kNB * kLd ];
+   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, i = lane & 31;
+   auto tile = [&]( int I, int J ) { return S + 32 * J * kLd + 32 * I; };
+   {
+      // 16-byte loads, all in flight before the first LDS store
+      constexpr int V = 16 / sizeof( T ), NV = kNB * kNB / V / 256;
+      typedef T vT __attribute__( ( ext_vector_type( V ) ) );
+      vT buf[ NV ];
 #pragma unroll
-   for ( int jj = 0; jj < RB; ++jj )
+      for ( int q = 0; q < NV; ++q ) {
+         const int v = q * 256 + tid, c = v / ( kNB / V ), r = ( v % ( kNB / V ) ) * V;
+         buf[ q ] = *reinterpret_cast<const vT*>( A + (long)c * lda + r );
+      }
 #pragma unroll
-      for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = A[ (long)( c0 + jj ) * lda + r0 + ii ];
+      for ( int q = 0; q < NV; ++q ) {
+         const int v = q * 256 + tid, c = v / ( kNB / V ), r = ( v % ( kNB / V ) ) * V;
+#pragma unroll
+         for ( int u = 0; u < V; ++u ) S[ c * kLd + r + u ] = buf[ q ][ u ];
+      }
+   }
+   __syncthreads();
+   bool big = false;
+   T a[ 32 ];
    if ( mode == 0 ) {
-      bool big = false;
-      for ( int jb = 0; jb < NB / RB; ++jb ) {
+      for ( int p = 0; p < 4; ++p ) {
+         T* Spp = tile( p, p );
+         if ( wave == 0 ) {
 #pragma unroll
-         for ( int jo = 0; jo < RB; ++jo ) {
-            const int j = jb * RB + jo, bsel = jo & 1;
-            if ( tx == jb ) {
+            for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
+            big |= leaf_lu( a, i );
+            if ( lane < 32 ) {
+               T* Ap = A + (long)( 32 * p ) * lda + 32 * p;
 #pragma unroll
-               for ( int ii = 0; ii < RB; ++ii ) colb[ bsel ][ r0 + ii ] = a[ ii ][ jo ];
-            }
-            if ( ty == jb ) {
-#pragma unroll
-               for ( int jj = 0; jj < RB; ++jj ) rowb[ bsel ][ c0 + jj ] = a[ jo ][ jj ];
-            }
-            __syncthreads();
-            if ( r0 + RB - 1 > j ) {  // this tile has rows below the pivot
-               const T piv = rowb[ bsel ][ j ];
-               const T rp = T( 1 ) / piv;
-               T l[ RB ], u[ RB ];
-#pragma unroll
-               for ( int ii = 0; ii < RB; ++ii ) {
-                  const T v = colb[ bsel ][ r0 + ii ];
-                  const bool below = r0 + ii > j;
-                  big |= below && !( tabs( v ) <= tabs( piv ) );
-                  l[ ii ] = below ? v * rp : T( 0 );
-               }
-#pragma unroll
-               for ( int jj = 0; jj < RB; ++jj ) u[ jj ] = c0 + jj > j ? rowb[ bsel ][ c0 + jj ] : T( 0 );
-#pragma unroll
-               for ( int jj = 0; jj < RB; ++jj )
-#pragma unroll
-                  for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - l[ ii ] * u[ jj ];
-               if ( tx == jb ) {
-#pragma unroll
-                  for ( int ii = 0; ii < RB; ++ii )
-                     if ( r0 + ii > j ) a[ ii ][ jo ] = l[ ii ];
+               for ( int c = 0; c < 32; ++c ) {
+                  Spp[ c * kLd + i ] = a[ c ];
+                  Ap[ (long)c * lda + i ] = a[ c ];
                }
             }
+         }
+         __syncthreads();
+         if ( wave == 0 ) {
+            leaf_inv_lower( a, i );
+            if ( lane < 32 ) {
+#pragma unroll
+               for ( int c = 0; c < 31; ++c )
+                  if ( c < i ) Spp[ c * kLd + i ] = a[ c ];
+            }
+         } else if ( wave == 1 ) {
+#pragma unroll
+            for ( int c = 0; c < 32; ++c ) a[ c ] = c >= i ? Spp[ c * kLd + i ] : T( 0 );
+            leaf_inv_upper( a, i );
+            if ( lane < 32 ) {
+#pragma unroll
+               for ( int c = 0; c < 32; ++c )
+                  if ( c >= i ) Spp[ c * kLd + i ] = a[ c ];
+            }
+         }
+         __syncthreads();
+         const int nr = 3 - p;
+         for ( int t = wave; t < 2 * nr; t += 4 ) {
+            v4 acc[ 2 ][ 2 ];
+            tile_zero<T>( acc );
+            if ( t < nr ) {  // L panel: A U_pp^-1 (the multipliers: checked against 1)
+               T* C = tile( p + 1 + t, p );
+               tile_mma<T, kOpFull, kOpUpper>( acc, C, Spp, T( 1 ), lane );
+               big |= tile_store<T>( acc, C, lane, true );
+            } else {  // U row: L_pp^-1 A
+               T* C = tile( p, p + 1 + t - nr );
+               tile_mma<T, kOpLowerUnit, kOpFull>( acc, Spp, C, T( 1 ), lane );
+               tile_store<T>( acc, C, lane, false );
+            }
+         }
+         __syncthreads();
+         for ( int t = wave; t < nr * nr; t += 4 ) {
+            const int I = p + 1 + t % nr, J = p + 1 + t / nr;
+            v4 acc[ 2 ][ 2 ];
+            tile_load<T>( acc, tile( I, J ), lane );
+            tile_mma<T, kOpFull, kOpFull>( acc, tile( I, p ), tile( p, J ), T( -1 ), lane );
+            tile_store<T>( acc, tile( I, J ), lane, false );
+         }
+         __syncthreads();
+      }
+      // (the diagonal tiles went out as they were factored)
+      {
+         // the off-diagonal tiles of the factors, 16-byte stores
+         constexpr int V = 16 / sizeof( T ), NV = kNB * kNB / V / 256;
+         typedef T vT __attribute__( ( ext_vector_type( V ) ) );
+#pragma unroll 4
+         for ( int q = 0; q < NV; ++q ) {
+            const int v = q * 256 + tid, c = v / ( kNB / V ), r = ( v % ( kNB / V ) ) * V;
+            if ( ( r >> 5 ) == ( c >> 5 ) ) continue;
+            vT x;
+#pragma unroll
+            for ( int u = 0; u < V; ++u ) x[ u ] = S[ c * kLd + r + u ];
+            *reinterpret_cast<vT*>( A + (long)c * lda + r ) = x;
          }
       }
       if ( big ) atomicOr( check, 1 );
+   } else {
+      T* Spp = tile( wave, wave );
 #pragma unroll
-      for ( int jj = 0; jj < RB; ++jj )
+      for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
+      leaf_inv_lower( a, i );
+      leaf_inv_upper( a, i );
+      if ( lane < 32 ) {
 #pragma unroll
-         for ( int ii = 0; ii < RB; ++ii ) A[ (long)( c0 + jj ) * lda + r0 + ii ] = a[ ii ][ jj ];
+         for ( int c = 0; c < 32; ++c ) Spp[ c * kLd + i ] = a[ c ];
+      }
    }
    __syncthreads();
-   for ( int sb = 0; sb < NB / RB; ++sb ) {
-#pragma unroll
-      for ( int so = 0; so < RB; ++so ) {
-         const int bsel = so & 1;
-         // U^-1 works on row ju = 127 - (8 sb + so): tile row/column offset 7 - so, tile index 15 - sb.
-         const int ju = NB - 1 - ( sb * RB + so ), uo = RB - 1 - so, ub = NB / RB - 1 - sb;
-         // L^-1 works on row jl = 8 sb + so: offset so, tile index sb.
-         const int jl = sb * RB + so;
-         if ( tx == ub ) {
-#pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) colb[ bsel ][ r0 + ii ] = a[ ii ][ uo ];
-         }
-         if ( ty == ub ) {
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj ) rowb[ bsel ][ c0 + jj ] = a[ uo ][ jj ];
-         }
-         if ( tx == sb ) {
-#pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) colL[ bsel ][ r0 + ii ] = a[ ii ][ so ];
-         }
-         if ( ty == sb ) {
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj ) rowL[ bsel ][ c0 + jj ] = a[ so ][ jj ];
-         }
-         __syncthreads();
-         const T d = T( 1 ) / rowb[ bsel ][ ju ];
-         // U^-1: rows i < ju get (i, ju) <- -u_{i,ju} d and (i, c > ju) -= u_{i,ju} X(ju, c).
-         if ( r0 < ju ) {
-            T uc[ RB ], xr[ RB ];
-#pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) uc[ ii ] = r0 + ii < ju ? colb[ bsel ][ r0 + ii ] : T( 0 );
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj ) xr[ jj ] = c0 + jj > ju ? rowb[ bsel ][ c0 + jj ] * d : T( 0 );
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj )
-#pragma unroll
-               for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - uc[ ii ] * xr[ jj ];
-            if ( tx == ub ) {
-#pragma unroll
-               for ( int ii = 0; ii < RB; ++ii )
-                  if ( r0 + ii < ju ) a[ ii ][ uo ] = -uc[ ii ] * d;
-            }
-         }
-         // row ju itself: X(ju, ju) = d, X(ju, c > ju) = Y(ju, c) d
-         if ( ty == ub ) {
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj ) {
-               const int c = c0 + jj;
-               a[ uo ][ jj ] = c == ju ? d : ( c > ju ? a[ uo ][ jj ] * d : a[ uo ][ jj ] );
-            }
-         }
-         // L^-1: rows i > jl get (i, jl) <- -l_{i,jl} and (i, c < jl) -= l_{i,jl} X(jl, c).
-         if ( r0 + RB - 1 > jl ) {
-            T lc[ RB ], xl[ RB ];
-#pragma unroll
-            for ( int ii = 0; ii < RB; ++ii ) lc[ ii ] = r0 + ii > jl ? colL[ bsel ][ r0 + ii ] : T( 0 );
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj ) xl[ jj ] = c0 + jj < jl ? rowL[ bsel ][ c0 + jj ] : T( 0 );
-#pragma unroll
-            for ( int jj = 0; jj < RB; ++jj )
-#pragma unroll
-               for ( int ii = 0; ii < RB; ++ii ) a[ ii ][ jj ] = a[ ii ][ jj ] - lc[ ii ] * xl[ jj ];
-            if ( tx == sb ) {
-#pragma unroll
-               for ( int ii = 0; ii < RB; ++ii )
-                  if ( r0 + ii > jl ) a[ ii ][ so ] = -lc[ ii ];
-            }
-         }
+   // 32 -> 64: waves 0, 1 the L blocks of tile pairs (0, 1) and (2, 3), waves 2, 3 the U blocks.
+   {
+      const int a0 = 2 * ( wave & 1 ), b0 = a0 + 1;
+      v4 acc[ 2 ][ 2 ];
+      if ( wave < 2 ) {
+         T* C = tile( b0, a0 );
+         tile_zero<T>( acc );
+         tile_mma<T, kOpFull, kOpLowerUnit>( acc, C, tile( a0, a0 ), T( 1 ), lane );
+         tile_store<T>( acc, C, lane, false );
+         tile_zero<T>( acc );
+         tile_mma<T, kOpLowerUnit, kOpFull>( acc, tile( b0, b0 ), C, T( -1 ), lane );
+         tile_store<T>( acc, C, lane, false );
+      } else {
+         T* C = tile( a0, b0 );
+         tile_zero<T>( acc );
+         tile_mma<T, kOpFull, kOpUpper>( acc, C, tile( b0, b0 ), T( 1 ), lane );
+         tile_store<T>( acc, C, lane, false );
+         tile_zero<T>( acc );
+         tile_mma<T, kOpUpper, kOpFull>( acc, tile( a0, a0 ), C, T( -1 ), lane );
+         tile_store<T>( acc, C, lane, false );
       }
    }
-   T* Li = inv;
-   T* Ui = inv + NB * NB;
-#pragma unroll
-   for ( int jj = 0; jj < RB; ++jj )
-#pragma unroll
-      for ( int ii = 0; ii < RB; ++ii ) {
-         const int r = r0 + ii, c = c0 + jj;
-         Li[ c * NB + r ] = r > c ? a[ ii ][ jj ] : ( r == c ? T( 1 ) : T( 0 ) );
-         Ui[ c * NB + r ] = r <= c ? a[ ii ][ jj ] : T( 0 );
+   __syncthreads();
+   // 64 -> 128.  Wave w: L tile (IL, JL) = (2 + w / 2, w % 2) and U tile (IU, JU) = (w / 2, 2 + w % 2).
+   {
+      const int IL = 2 + ( wave >> 1 ), JL = wave & 1, IU = wave >> 1, JU = 2 + ( wave & 1 );
+      v4 accL[ 2 ][ 2 ], accU[ 2 ][ 2 ];
+      // T = C A^-1 (L), T = C B^-1 (U)
+      tile_zero<T>( accL );
+      tile_zero<T>( accU );
+      if ( JL == 0 ) {
+         tile_mma<T, kOpFull, kOpLowerUnit>( accL, tile( IL, 0 ), tile( 0, 0 ), T( 1 ), lane );
+         tile_mma<T, kOpFull, kOpFull>( accL, tile( IL, 1 ), tile( 1, 0 ), T( 1 ), lane );
+         tile_mma<T, kOpFull, kOpUpper>( accU, tile( IU, 2 ), tile( 2, 2 ), T( 1 ), lane );
+      } else {
+         tile_mma<T, kOpFull, kOpLowerUnit>( accL, tile( IL, 1 ), tile( 1, 1 ), T( 1 ), lane );
+         tile_mma<T, kOpFull, kOpFull>( accU, tile( IU, 2 ), tile( 2, 3 ), T( 1 ), lane );
+         tile_mma<T, kOpFull, kOpUpper>( accU, tile( IU, 3 ), tile( 3, 3 ), T( 1 ), lane );
       }
+      __syncthreads();
+      tile_store<T>( accL, tile( IL, JL ), lane, false );
+      tile_store<T>( accU, tile( IU, JU ), lane, false );
+      __syncthreads();
+      // X = -B^-1 T (L), X = -A^-1 T (U)
+      tile_zero<T>( accL );
+      tile_zero<T>( accU );
+      if ( IL == 2 ) {
+         tile_mma<T, kOpLowerUnit, kOpFull>( accL, tile( 2, 2 ), tile( 2, JL ), T( -1 ), lane );
+         tile_mma<T, kOpUpper, kOpFull>( accU, tile( 0, 0 ), tile( 0, JU ), T( -1 ), lane );
+         tile_mma<T, kOpFull, kOpFull>( accU, tile( 0, 1 ), tile( 1, JU ), T( -1 ), lane );
+      } else {
+         tile_mma<T, kOpFull, kOpFull>( accL, tile( 3, 2 ), tile( 2, JL ), T( -1 ), lane );
+         tile_mma<T, kOpLowerUnit, kOpFull>( accL, tile( 3, 3 ), tile( 3, JL ), T( -1 ), lane );
+         tile_mma<T, kOpUpper, kOpFull>( accU, tile( 1, 1 ), tile( 1, JU ), T( -1 ), lane );
+      }
+      __syncthreads();
+      tile_store<T>( accL, tile( IL, JL ), lane, false );
+      tile_store<T>( accU, tile( IU, JU ), lane, false );
+   }
+   __syncthreads();
+   T* Li = inv;
+   T* Ui = inv + kNB * kNB;
+   constexpr int V = 16 / sizeof( T ), NV = kNB * kNB / V / 256;
+   typedef T vT __attribute__( ( ext_vector_type( V ) ) );
+#pragma unroll 4
+   for ( int q = 0; q < NV; ++q ) {
+      const int v = q * 256 + tid, c = v / ( kNB / V ), r = ( v % ( kNB / V ) ) * V;
+      vT xl, xu;
+#pragma unroll
+      for ( int u = 0; u < V; ++u ) {
+         const T x = S[ c * kLd + r + u ];
+         xl[ u ] = r + u > c ? x : ( r + u == c ? T( 1 ) : T( 0 ) );
+         xu[ u ] = r + u <= c ? x : T( 0 );
+      }
+      *reinterpret_cast<vT*>( Li + c * kNB + r ) = xl;
+      *reinterpret_cast<vT*>( Ui + c * kNB + r ) = xu;
+   }
 }
 
 // ------------------------------------------------------------------ partial pivoting (slow path)
@@ -649,62 +855,70 @@ template void launch_gemm<double>( hipStream_t, int, int, int, double, const dou
 template void launch_gemm<float>( hipStream_t, int, int, int, float, const float*, long, const float*, long, float, float*,
                                   long, int* );
 
-// The blocked factorization without exchanges (the fast path), with a look-ahead of one block
-// column: the trailing update of step k is split into the next block column (on st, where the
-// diagonal block and the panels of step k+1 follow at once) and the rest (on st2), so the
-// one-workgroup diagonal factorization of step k+1 runs while the GPU is busy with the bulk of
-// step k's update.  ev_u and ev_g order the two streams (U panel k before its update on st2; the
-// update on st2 before U panel k+1, which reads the rows it writes).  check is set when partial
-// pivoting would have exchanged rows somewhere; the caller must then rerun lu_pivoted on the
-// original matrix.
+// The blocked factorization without exchanges (the fast path).  Block columns are taken in pairs
+// (npad is a multiple of 256): the two 128-column blocks of a pair are factored one after the other
+// (diagonal block, the panel below as A U^-1, and in between the update of the second block
+// column by the first), then the two U row panels are formed and the trailing matrix gets one
+// rank-256 update, which halves the passes over the trailing matrix of a rank-128 schedule.  The
+// trailing update is split: the next pair of block columns on st, so the next pair's diagonal
+// blocks start at once, and the rest on st2, overlapping them.  ev_u and ev_g order the streams
+// (the U panels of the pair before the update on st2; that update before the next pair's U
+// panels, which read and write rows it updates).  check is set when partial pivoting would have
+// exchanged rows somewhere; the caller must then rerun lu_pivoted on the original matrix.
 template <typename T>
 void lu_nopiv( hipStream_t st, hipStream_t st2, hipEvent_t ev_u, hipEvent_t ev_g, int npad, T* A, long lda, T* inv,
                int* check ) {
    const int nb = kNB, nblk = npad / nb;
+   auto blk = [&]( int r, int c ) { return A + (long)c * nb * lda + (long)r * nb; };
+   auto inv_l = [&]( int k ) { return inv + (long)k * 2 * nb * nb; };
+   auto inv_u = [&]( int k ) { return inv + (long)k * 2 * nb * nb + nb * nb; };
    auto diag = [&]( int k ) {
-      const int kk = k * nb;
-      ProfScope prof( kKDiag, 2.0 * nb * nb * nb, st );  // LU 2/3 nb^3 + two triangular inverses 2/3 nb^3
-      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, A + (long)kk * lda + kk, lda,
-                          inv + (long)k * 2 * nb * nb, check );
+      ProfScope prof( kKDiag, 4.0 / 3.0 * nb * nb * nb, st );  // LU 2/3 nb^3 + two triangular inverses nb^3 / 3 each
+      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, blk( k, k ), lda, inv_l( k ), check );
    };
-   auto lpanel = [&]( int k ) {  // A_below <- A_below U^-1 (checks |l| <= 1)
-      const int kk = k * nb, rest = npad - kk - nb;
-      T* Akk = A + (long)kk * lda + kk;
-      launch_gemm<T>( st, rest, nb, nb, T( 1 ), Akk + nb, lda, inv + (long)k * 2 * nb * nb + nb * nb, nb, T( 0 ), Akk + nb, lda,
-                      check );
+   auto lpanel = [&]( int k ) {  // the panel below the diagonal block: A U^-1 (checks |l| <= 1)
+      launch_gemm<T>( st, npad - ( k + 1 ) * nb, nb, nb, T( 1 ), blk( k + 1, k ), lda, inv_u( k ), nb, T( 0 ),
+                      blk( k + 1, k ), lda, check );
    };
-   auto upanel = [&]( int k ) {  // A_right <- L^-1 A_right
-      const int kk = k * nb, rest = npad - kk - nb;
-      T* Akk = A + (long)kk * lda + kk;
-      launch_gemm<T>( st, nb, rest, nb, T( 1 ), inv + (long)k * 2 * nb * nb, nb, Akk + (long)nb * lda, lda, T( 0 ),
-                      Akk + (long)nb * lda, lda, nullptr );
+   auto urow = [&]( int k, int c0, int ncols ) {  // U row panel: L^-1 A over block columns c0..
+      launch_gemm<T>( st, nb, ncols, nb, T( 1 ), inv_l( k ), nb, blk( k, c0 ), lda, T( 0 ), blk( k, c0 ), lda, nullptr );
    };
-   diag( 0 );
-   if ( nblk == 1 ) return;
-   lpanel( 0 );
-   upanel( 0 );
-   for ( int k = 0; k + 1 < nblk; ++k ) {
-      const int kk = k * nb, rest = npad - kk - nb;
-      T* Akk = A + (long)kk * lda + kk;
-      T* L = Akk + nb;
-      T* U = Akk + (long)nb * lda;
-      // next block column of the trailing matrix, on st
-      launch_gemm<T>( st, rest, nb, nb, T( -1 ), L, lda, U, lda, T( 1 ), U + nb, lda, nullptr );
-      // the rest of the trailing update, on st2, after U panel k
-      if ( rest > nb ) {
+   bool pending = false;  // an update on st2 that the next pair's U panels must wait for
+   for ( int k0 = 0; k0 < nblk; k0 += 2 ) {
+      const int k1 = k0 + 1;
+      if ( k1 >= nblk ) {  // a last single block (npad an odd multiple of 128)
+         diag( k0 );
+         break;
+      }
+      diag( k0 );
+      lpanel( k0 );
+      urow( k0, k1, nb );
+      launch_gemm<T>( st, npad - k1 * nb, nb, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, k1 ), lda, T( 1 ), blk( k1, k1 ), lda,
+                      nullptr );
+      diag( k1 );
+      if ( k1 + 1 == nblk ) break;
+      lpanel( k1 );
+      const int c2 = k1 + 1, rest = npad - c2 * nb;
+      if ( pending ) BH_HIP( hipStreamWaitEvent( st, ev_g, 0 ) );
+      pending = false;
+      urow( k0, c2, rest );
+      launch_gemm<T>( st, nb, rest, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( k1, c2 ), lda, nullptr );
+      urow( k1, c2, rest );
+      // the rank-256 trailing update: L = the two panels below, U = the two row panels
+      const int nnext = std::min( 2 * nb, rest );
+      launch_gemm<T>( st, rest, nnext, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( c2, c2 ), lda,
+                      nullptr );
+      if ( rest > nnext ) {
          BH_HIP( hipEventRecord( ev_u, st ) );
          BH_HIP( hipStreamWaitEvent( st2, ev_u, 0 ) );
-         launch_gemm<T>( st2, rest, rest - nb, nb, T( -1 ), L, lda, U + (long)nb * lda, lda, T( 1 ), U + (long)nb * lda + nb,
-                         lda, nullptr );
+         launch_gemm<T>( st2, rest, rest - nnext, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, c2 + 2 ), lda, T( 1 ),
+                         blk( c2, c2 + 2 ), lda, nullptr );
          BH_HIP( hipEventRecord( ev_g, st2 ) );
-      }
-      diag( k + 1 );
-      if ( k + 2 < nblk ) {
-         lpanel( k + 1 );
-         BH_HIP( hipStreamWaitEvent( st, ev_g, 0 ) );
-         upanel( k + 1 );
+         pending = true;
       }
    }
+   // the caller's later work on st must see st2's last update
+   if ( pending ) BH_HIP( hipStreamWaitEvent( st, ev_g, 0 ) );
 }
 template void lu_nopiv<double>( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, double*, long, double*, int* );
 template void lu_nopiv<float>( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, float*, long, float*, int* );
