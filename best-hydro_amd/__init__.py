@@ -64,6 +64,18 @@ class BhStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BhHmStats(C.Structure):
+    _fields_ = [("depth", C.c_int), ("nleaves", C.c_int), ("leaf_min", C.c_int), ("leaf_max", C.c_int),
+                ("n_dense", C.c_long), ("n_lowrank", C.c_long), ("n_small", C.c_long), ("n_capped", C.c_long),
+                ("bytes_dense", C.c_double), ("bytes_lowrank", C.c_double), ("kmax", C.c_int), ("kmean", C.c_double),
+                ("kernel_blocks", C.c_double), ("t_tree", C.c_double), ("t_fill", C.c_double), ("t_pre", C.c_double),
+                ("iters_min", C.c_int), ("iters_max", C.c_int), ("steps", C.c_int), ("relres", C.c_double),
+                ("t_solve", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def lib():
     """The loaded libbesthip.so (raises if it has not been built)."""
     global _LIB
@@ -122,6 +134,14 @@ def lib():
     L.bhg_factor_deflated.restype = i
     L.bhg_besolv.argtypes = [vp, i, _dp, lg]
     L.bhg_besolv.restype = i
+    L.bhg_set_method.argtypes = [vp, i, d, i, d]
+    L.bhg_set_method.restype = i
+    L.bhg_get_hm_stats.argtypes = [vp, C.POINTER(BhHmStats)]
+    L.bhg_get_hm_stats.restype = i
+    L.bhg_hm_plan.argtypes = [i, _dp, _dp, _dp, _dp, i, d, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    L.bhg_hm_plan.restype = i
+    L.bhg_hm_apply.argtypes = [vp, i, _dp, lg, _dp, lg]
+    L.bhg_hm_apply.restype = i
     _LIB = L
     return L
 
@@ -215,6 +235,26 @@ class Solver:
     def set_refinement(self, tol, maxit=30):
         self.L.bhg_set_refinement(self.h, float(tol), int(maxit))
 
+    # -- the H-matrix path
+    def set_method(self, method="dense", eps=1e-6, single=False, tol=0.0):
+        """'dense' (LU) or 'hmatrix' (cross-approximated blocks + GMRES) for the next solve_mesh."""
+        m = {"dense": 0, "hmatrix": 1}[method]
+        self._check(self.L.bhg_set_method(self.h, m, float(eps), int(bool(single)), float(tol)), "set_method")
+
+    def hm_stats(self):
+        s = BhHmStats()
+        self.L.bhg_get_hm_stats(self.h, C.byref(s))
+        return s.as_dict()
+
+    def hm_apply(self, x):
+        """(GR + deflation) x with the built H-matrix."""
+        x = np.asfortranarray(x, dtype=np.float64)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        y = np.zeros(3 * self.n * nrhs)
+        self._check(self.L.bhg_hm_apply(self.h, nrhs, x.reshape(-1, order="F").copy(), 3 * self.n, y, 3 * self.n),
+                    "hm_apply")
+        return y.reshape(x.shape, order="F")
+
     # -- the dense path piece by piece (the besolve module's twins)
     def load_prepared(self, n, v1, v2, v3, area, afac, cg=1.0):
         """The mesh as bhm_create receives it: flat column-major shifted vertices, corrected areas."""
@@ -264,6 +304,18 @@ class Solver:
         flat = b.reshape(-1, order="F")
         self._check(self.L.bhg_besolv(self.h, nrhs, flat, 3 * self.n), "besolv")
         return flat.reshape(b.shape, order="F")
+
+
+def hm_plan(v1, v2, v3, x, leaf=32, eta=2.0):
+    """Cluster and block trees on the host: depth, leaf clusters, smallest and largest leaf, dense
+    leaf blocks, admissible blocks (low-rank), small admissible blocks.  v1, v2, v3, x: NAR x 3."""
+    out = np.zeros(7, dtype=np.int64)
+    n = len(v1)
+    rc = lib().bhg_hm_plan(n, _col(v1), _col(v2), _col(v3), _col(x), int(leaf), float(eta), out)
+    if rc != 0:
+        raise HipError("hm_plan failed")
+    keys = ("depth", "nleaves", "leaf_min", "leaf_max", "n_dense", "n_lowrank", "n_small")
+    return {k: int(v) for k, v in zip(keys, out)}
 
 
 class _HM:

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -15,6 +16,9 @@ namespace bhg {
 // the 160 KiB per CU) and every GEMM tile edge is a multiple of it, so matrices are padded to a
 // multiple of kNB with an identity block.
 constexpr int kNB = 128;
+
+// Right-hand sides per solve call at most.
+constexpr int kMaxRhs = 16;
 
 // Per-source-triangle record used by the pair kernel: the vertices, the corrected area and the
 // source-only quantities of the quadrature (edge vectors, their dot products, sqrt(d2121)).
@@ -115,6 +119,28 @@ inline void hip_check( hipError_t e, const char* what, const char* file, int lin
       throw std::runtime_error( buf );
    }
 }
+
+// A device allocation owned by the host object that holds it.
+template <typename T>
+struct DevBuf {
+   T* p = nullptr;
+   size_t n = 0;
+   ~DevBuf() { release(); }
+   void release() {
+      if ( p ) (void)hipFree( p );
+      p = nullptr;
+      n = 0;
+   }
+   // Grows to at least count elements (contents are not kept).
+   T* ensure( size_t count ) {
+      if ( count > n ) {
+         release();
+         hip_check( hipMalloc( &p, std::max<size_t>( count, 1 ) * sizeof( T ) ), "hipMalloc", __FILE__, __LINE__ );
+         n = count;
+      }
+      return p;
+   }
+};
 
 }  // namespace bhg
 

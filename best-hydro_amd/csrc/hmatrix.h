@@ -1,0 +1,81 @@
+// The hierarchical-matrix path of the solver: the counterpart of hmatrix/src (cluster tree, block
+// tree, low-rank blocks by adaptive cross approximation, the H-matrix product, GMRES) with the
+// blocks, the products and the Krylov vectors in device memory.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bh_common.h"
+
+namespace bhg {
+
+struct HmOptions {
+   double eps = 1e-6;          // relative accuracy of the low-rank blocks
+   double eta = 2.0;           // admissibility: min(diam t, diam s) <= eta dist(t, s)
+   double aca_factor = 0.1;    // the cross approximation runs to aca_factor * eps
+   int leaf = 32;              // triangles per leaf cluster at most
+   long dense_small = 1024;    // admissible blocks of at most this many triangle pairs are kept dense
+   int kcap = 192;             // rank at which a cross approximation gives up (the block is then kept dense)
+   int single = 0;             // 1: blocks stored in FP32 (arithmetic in FP64)
+   double tol = 0.0;           // GMRES relative residual; 0: aca_factor * eps
+   int restart = 30;           // GMRES restart length
+   int maxsteps = 500;         // GMRES steps (batched products) per solve
+   double ws_bytes = 24e9;     // device work space of one batch of cross approximations
+};
+
+struct HmStats {
+   int depth = 0, nleaves = 0, leaf_min = 0, leaf_max = 0;
+   long n_dense = 0;           // inadmissible leaf blocks (leaf cluster pairs)
+   long n_lowrank = 0;         // admissible blocks stored as U V^T
+   long n_small = 0;           // admissible blocks small enough to be kept dense
+   long n_capped = 0;          // admissible blocks whose approximation reached kcap (kept dense)
+   double bytes_dense = 0, bytes_lowrank = 0;
+   int kmax = 0;
+   double kmean = 0;
+   double kernel_blocks = 0;   // 3x3 kernel blocks evaluated by the build
+   double t_tree = 0, t_fill = 0, t_pre = 0;   // seconds
+   int iters_min = 0, iters_max = 0, steps = 0;
+   double relres = 0, t_solve = 0;
+};
+
+// The geometry: vertices (v1, v2, v3 as BEST's NAR x 3 column-major arrays, one after the other)
+// and incenters (NAR x 3, column-major) on the host; the per-triangle records, incenters, self
+// blocks and the deflation (XMU, the surface of each triangle, GDEF per surface) on the device.
+struct HmInput {
+   int n = 0;
+   double cg = 1.0;
+   const double* h_v = nullptr;
+   const double* h_xc = nullptr;
+   const double* d_src = nullptr;
+   const double* d_xc = nullptr;
+   const double* d_selfb = nullptr;
+   const double* d_xmu = nullptr;
+   const int* d_comp = nullptr;
+   const double* d_gdef = nullptr;
+   int ndef = 1;
+};
+
+// Cluster and block trees alone (host): the counts the build would log.
+HmStats hm_plan( int n, const double* h_v, const double* h_xc, const HmOptions& opt );
+
+struct HMatrix;
+struct HMatrixDeleter {
+   void operator()( HMatrix* h ) const;
+};
+using HMatrixPtr = std::unique_ptr<HMatrix, HMatrixDeleter>;
+
+// Builds the H-matrix of GR + sum_k gdef[k] xmu_k xmu_k^T and the GMRES preconditioner.
+HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, HmStats* stats,
+                     std::vector<std::string>* log );
+
+// Solves for nrhs (<= 16) right-hand sides: b and x are device arrays in BEST's unknown order
+// (3 n x nrhs, column-major, leading dimensions ldb and ldx).  Throws if GMRES does not converge.
+void hm_solve( HMatrix& h, hipStream_t st, int nrhs, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
+               std::vector<std::string>* log );
+
+// y = (GR + deflation) x on the device, BEST's order (for checks).
+void hm_apply( HMatrix& h, hipStream_t st, int nrhs, const double* d_x, long ldx, double* d_y, long ldy );
+
+}  // namespace bhg

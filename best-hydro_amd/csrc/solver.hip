@@ -19,6 +19,7 @@
 
 #include "../../include/best_hip.h"
 #include "bh_common.h"
+#include "hmatrix.h"
 #include "transport.h"
 
 namespace bhg {
@@ -54,30 +55,7 @@ void ir_update( hipStream_t, int, int, double*, long, const float*, long, double
 void convert_d2f( hipStream_t, long, const double*, float* );
 void convert_f2d( hipStream_t, long, const float*, double* );
 
-// ------------------------------------------------------------------ device buffers
 
-template <typename T>
-struct DevBuf {
-   T* p = nullptr;
-   size_t n = 0;
-   ~DevBuf() { release(); }
-   void release() {
-      if ( p ) (void)hipFree( p );
-      p = nullptr;
-      n = 0;
-   }
-   // Grows to at least count elements (contents are not kept).
-   T* ensure( size_t count ) {
-      if ( count > n ) {
-         release();
-         BH_HIP( hipMalloc( &p, std::max<size_t>( count, 1 ) * sizeof( T ) ) );
-         n = count;
-      }
-      return p;
-   }
-};
-
-constexpr int kMaxRhs = 16;
 // Order of the diagonal super-blocks whose inverses the triangular solves use; the matrix is
 // padded to a multiple of it.
 constexpr int kSuper = 1024;
@@ -127,7 +105,12 @@ struct bhg_solver {
    bhg::DevBuf<int> ipiv, flags;
    bhg::DevBuf<double> pval;
    bhg::DevBuf<int> pidx;
-   int factored = 0;          // 0, 64 or 32
+   int factored = 0;          // 0, 64 or 32; -1: the H-matrix is built (method 1)
+   // H-matrix path (method 1)
+   int method = 0;            // 0 dense LU, 1 H-matrix + GMRES
+   bhg::HmOptions hm_opt;
+   bhg::HmStats hm_stats;
+   bhg::HMatrixPtr hm;
    bool pivoted = false;      // the slow path ran (rows were exchanged)
    bool deflated = true;      // the factored matrix carries the deflation
    // vectors
@@ -212,6 +195,7 @@ int read_flag( bhg_solver* s ) {
 // copy (a64 is kept for the refinement residuals).  Partial pivoting falls back to the slow path
 // when the exchange-free factorization is not GEPP's (see lu.hip).
 void factor( bhg_solver* s, int precision ) {
+   s->hm.reset();
    const size_t np = s->npad;
    const int nblk = s->npad / kNB;
    s->ipiv.ensure( np );
@@ -251,6 +235,35 @@ void factor( bhg_solver* s, int precision ) {
    s->factored = precision;
 }
 
+// The H-matrix of the deflated GR and its preconditioner, from the resident mesh (prepared
+// vertices, incenters, self blocks, XMU, surfaces and weights).
+void hm_factor( bhg_solver* s ) {
+   s->hm.reset();
+   s->a64.release();
+   s->a32.release();
+   std::vector<double> hv( 9 * (size_t)s->n ), hx( 3 * (size_t)s->n );
+   BH_HIP( hipMemcpyAsync( hv.data(), s->v.p, hv.size() * sizeof( double ), hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipMemcpyAsync( hx.data(), s->xc.p, hx.size() * sizeof( double ), hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipStreamSynchronize( s->st ) );
+   HmInput in;
+   in.n = s->n;
+   in.cg = s->cg;
+   in.h_v = hv.data();
+   in.h_xc = hx.data();
+   in.d_src = s->src.p;
+   in.d_xc = s->xc.p;
+   in.d_selfb = s->selfb.p;
+   in.d_xmu = s->xmu.p;
+   in.d_comp = s->comp.p;
+   in.d_gdef = s->gdef.p;
+   in.ndef = s->ndef;
+   std::vector<std::string> lines;
+   s->hm = hm_build( s->st, in, s->hm_opt, &s->hm_stats, &lines );
+   for ( auto& l : lines ) s->log.push_back( std::move( l ) );
+   s->factored = -1;
+   s->pivoted = false;
+}
+
 void size_vectors( bhg_solver* s ) {
    const size_t np = s->npad;
    const size_t need = np * kMaxRhs;
@@ -280,6 +293,12 @@ void solve_device( bhg_solver* s, int nrhs ) {
    const long np = s->npad;
    s->stats.ir_iterations = 0;
    s->stats.ir_correction = 0.0;
+   if ( s->factored == -1 ) {
+      std::vector<std::string> lines;
+      hm_solve( *s->hm, s->st, nrhs, s->b.p, np, s->x.p, np, &s->hm_stats, &lines );
+      for ( auto& l : lines ) s->log.push_back( std::move( l ) );
+      return;
+   }
    if ( s->factored == 64 ) {
       to_interleaved<double, double>( s->st, s->npad, nrhs, s->b.p, np, s->r.p );
       lu_solve<double>( s->st, s->npad, s->n3, kSuper, s->a64.p, np, s->xl64.p, s->xu64.p, s->ipiv.p, s->pivoted, s->r.p,
@@ -528,9 +547,14 @@ int bhg_solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, i
       bhg::self_terms( s, true, false );
       s->mesh_ready = true;
       T.mark( s->st, 1 );
-      bhg::assemble( s, true );
-      T.mark( s->st, 2 );
-      bhg::factor( s, precision );
+      if ( s->method == 1 ) {  // the H-matrix build is the factorization phase
+         T.mark( s->st, 2 );
+         bhg::hm_factor( s );
+      } else {
+         bhg::assemble( s, true );
+         T.mark( s->st, 2 );
+         bhg::factor( s, precision );
+      }
       T.mark( s->st, 3 );
       bhg::size_vectors( s );
       const int nrhs = ivisco ? 11 : 6;
@@ -632,6 +656,82 @@ int bhg_set_refinement( bhg_solver* s, double tol, int maxit ) {
    s->ir_tol = tol;
    s->ir_maxit = maxit;
    return 0;
+}
+
+int bhg_set_method( bhg_solver* s, int method, double eps, int single, double tol ) {
+   if ( !s || ( method != 0 && method != 1 ) ) return 1;
+   s->method = method;
+   s->hm_opt.eps = eps > 0.0 ? eps : 1e-6;
+   s->hm_opt.single = single ? 1 : 0;
+   s->hm_opt.tol = tol;
+   return 0;
+}
+
+int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out ) {
+   if ( !s || !out ) return 1;
+   const bhg::HmStats& h = s->hm_stats;
+   out->depth = h.depth;
+   out->nleaves = h.nleaves;
+   out->leaf_min = h.leaf_min;
+   out->leaf_max = h.leaf_max;
+   out->n_dense = h.n_dense;
+   out->n_lowrank = h.n_lowrank;
+   out->n_small = h.n_small;
+   out->n_capped = h.n_capped;
+   out->bytes_dense = h.bytes_dense;
+   out->bytes_lowrank = h.bytes_lowrank;
+   out->kmax = h.kmax;
+   out->kmean = h.kmean;
+   out->kernel_blocks = h.kernel_blocks;
+   out->t_tree = h.t_tree;
+   out->t_fill = h.t_fill;
+   out->t_pre = h.t_pre;
+   out->iters_min = h.iters_min;
+   out->iters_max = h.iters_max;
+   out->steps = h.steps;
+   out->relres = h.relres;
+   out->t_solve = h.t_solve;
+   return 0;
+}
+
+int bhg_hm_plan( int n, const double* v1, const double* v2, const double* v3, const double* x, int leaf, double eta,
+                 long* out ) {
+   try {
+      std::vector<double> hv( 9 * (size_t)n );
+      std::memcpy( hv.data(), v1, 3 * (size_t)n * sizeof( double ) );
+      std::memcpy( hv.data() + 3 * (size_t)n, v2, 3 * (size_t)n * sizeof( double ) );
+      std::memcpy( hv.data() + 6 * (size_t)n, v3, 3 * (size_t)n * sizeof( double ) );
+      bhg::HmOptions opt;
+      opt.leaf = leaf;
+      opt.eta = eta;
+      const bhg::HmStats st = bhg::hm_plan( n, hv.data(), x, opt );
+      out[ 0 ] = st.depth;
+      out[ 1 ] = st.nleaves;
+      out[ 2 ] = st.leaf_min;
+      out[ 3 ] = st.leaf_max;
+      out[ 4 ] = st.n_dense;
+      out[ 5 ] = st.n_lowrank;
+      out[ 6 ] = st.n_small;
+      return 0;
+   } catch ( ... ) {
+      return 1;
+   }
+}
+
+int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y, long ldy ) {
+   return guarded( s, "bhg_hm_apply", [&] {
+      if ( !s->hm ) throw std::runtime_error( "no H-matrix" );
+      if ( nrhs < 1 || nrhs > bhg::kMaxRhs ) throw std::runtime_error( "between 1 and 16 columns per call" );
+      bhg::set_device( s );
+      const long np = s->npad;
+      bhg::size_vectors( s );
+      BH_HIP( hipMemcpy2DAsync( s->b.p, np * sizeof( double ), x, ldx * sizeof( double ), s->n3 * sizeof( double ), nrhs,
+                                hipMemcpyHostToDevice, s->st ) );
+      bhg::hm_apply( *s->hm, s->st, nrhs, s->b.p, np, s->x.p, np );
+      BH_HIP( hipMemcpy2DAsync( y, ldy * sizeof( double ), s->x.p, np * sizeof( double ), s->n3 * sizeof( double ), nrhs,
+                                hipMemcpyDeviceToHost, s->st ) );
+      BH_HIP( hipStreamSynchronize( s->st ) );
+   } );
 }
 
 // The prepared mesh of bhm_create (vertices already shifted, areas corrected).
@@ -758,6 +858,9 @@ int bhg_besolv( bhg_solver* s, int nrhs, double* bh, long ldb ) {
 static bhg_solver* g_solver = nullptr;
 static std::deque<std::string> g_log;
 static int g_precision = 32;
+// Above this many unknowns bhm_build takes the H-matrix path (the reference's NHMAT, 10,000
+// triangles, best/besolve.f90).
+static const int kDenseMaxUnknowns = 30000;
 
 static void g_drain() {
    if ( !g_solver ) return;
@@ -814,6 +917,32 @@ int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* g
    if ( !g_solver ) {
       g_log.emplace_back( "error: bhm_build before bhm_create" );
       return 1;
+   }
+   // The H-matrix path above kDenseMaxUnknowns (BEST_HIP_SOLVER=dense or hmatrix overrides).
+   const char* which = std::getenv( "BEST_HIP_SOLVER" );
+   const bool use_hm = which ? std::string( which ) == "hmatrix" : g_solver->n3 > kDenseMaxUnknowns;
+   if ( use_hm ) {
+      bhg_solver* s = g_solver;
+      const int rc = guarded( s, "bhm_build", [&] {
+         const char* tol = std::getenv( "BEST_HMATRIX_GMRES_TOL" );
+         s->method = 1;
+         s->hm_opt.eps = eps > 0.0 ? eps : 1e-4;
+         s->hm_opt.single = single ? 1 : 0;
+         s->hm_opt.tol = tol ? std::atof( tol ) : 0.0;
+         if ( direct ) s->say( "the direct H-LU (-hlu) is not on the GPU path: GMRES runs to the same tolerance instead" );
+         s->ndef = std::max( ndef, 1 );
+         s->gdef.ensure( s->ndef );
+         BH_HIP( hipMemcpyAsync( s->gdef.p, gdef, s->ndef * sizeof( double ), hipMemcpyHostToDevice, s->st ) );
+         std::vector<int> ones( s->n, 1 );
+         BH_HIP( hipMemcpyAsync( s->comp.p, ( comp && s->ndef > 1 ) ? comp : ones.data(), s->n * sizeof( int ),
+                                 hipMemcpyHostToDevice, s->st ) );
+         BH_HIP( hipMemcpyAsync( s->xmu.p, xmu, 3 * (size_t)s->n * sizeof( double ), hipMemcpyHostToDevice, s->st ) );
+         BH_HIP( hipStreamSynchronize( s->st ) );
+         bhg::hm_factor( s );
+         bhg::size_vectors( s );
+      } );
+      g_drain();
+      return rc;
    }
    const int prec = single ? 32 : g_precision;
    const size_t np = g_solver->npad;

@@ -138,6 +138,40 @@ int bhg_factor_deflated( bhg_solver* s, int ndef, const double* gdef, const int*
 /* BESOLV twin (best/besolve.f90:95-134): b (3n x nrhs, leading dimension ldb) in place. */
 int bhg_besolv( bhg_solver* s, int nrhs, double* b, long ldb );
 
+/* ------------------------------------------------------------------ the H-matrix path */
+
+/* Diagnostics of the last H-matrix build and solve (the figures hmatrix/src/api.cpp:380-386 and
+ * :722-735 log). */
+typedef struct bhg_hm_stats {
+   int depth, nleaves, leaf_min, leaf_max; /* cluster tree */
+   long n_dense, n_lowrank, n_small, n_capped; /* leaf blocks: inadmissible, U V^T, small admissible
+                                                 (dense), approximations that reached the rank cap */
+   double bytes_dense, bytes_lowrank;
+   int kmax;
+   double kmean, kernel_blocks; /* ranks; 3x3 kernel blocks evaluated by the build */
+   double t_tree, t_fill, t_pre; /* seconds */
+   int iters_min, iters_max, steps; /* GMRES iterations per right-hand side, batched products */
+   double relres, t_solve;
+} bhg_hm_stats;
+
+/* method 0: dense LU (default); 1: H-matrix + GMRES (hmatrix/src/api.cpp:585-735) for the next
+ * bhg_solve_mesh: low-rank blocks by cross approximation at eps (BEST's -he), stored in FP32 with
+ * single = 1 (-hs), GMRES to the relative residual tol (0: eps / 10, as BEST_HMATRIX_GMRES_TOL's
+ * default, hmatrix/src/api.cpp:840).  bhm_build takes this path above 30,000 unknowns, or always
+ * with BEST_HIP_SOLVER=hmatrix. */
+int bhg_set_method( bhg_solver* s, int method, double eps, int single, double tol );
+int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out );
+
+/* The cluster and block trees alone, on the host (hmatrix/src/cluster.cpp:91-112,
+ * hmatrix/src/hmatrix.cpp:27-54): out[0..6] = depth, leaf clusters, smallest and largest leaf,
+ * inadmissible leaf blocks, admissible blocks above 1,024 triangle pairs, admissible blocks of at
+ * most 1,024.  v1, v2, v3, x as BEST's NAR x 3 column-major arrays. */
+int bhg_hm_plan( int n, const double* v1, const double* v2, const double* v3, const double* x, int leaf, double eta,
+                 long* out );
+
+/* y = (GR + deflation) x with the built H-matrix (hmatrix/src/hmatrix.cpp:441-471), BEST's order. */
+int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y, long ldy );
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,132 @@
+"""The H-matrix path on the GPU (cross-approximated blocks, the block-tree product, GMRES) against
+the dense FP64 oracle and against the reference's own H-matrix library (oracle/_ref).
+
+Bars: the product (GR + deflation) x within 10 eps of the dense product; GMRES to its relative
+residual; D_t, D_r and [eta] within 1e-5 (eps = 1e-6) of the dense answers; the reference library
+at the same eps agrees with the dense answers to the same order, so the two H-matrix answers agree
+to twice that."""
+import os
+
+import numpy as np
+import pytest
+from conftest import GOLDEN, rel
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-6
+
+
+@pytest.fixture(scope="module")
+def solver(bh):
+    s = bh.Solver(0)
+    yield s
+    s.close()
+
+
+def _prepared(oracle, fname):
+    v1, v2, v3, ta, tv = oracle.read_best(os.path.join(GOLDEN, fname))
+    return oracle.prep(v1, v2, v3, 1, ta), (v1, v2, v3, ta, tv)
+
+
+def test_product_matches_dense(bh, oracle, solver):
+    from best_hydro_amd import meshes as M
+    path = os.path.join(GOLDEN, "2eu7_02562")
+    tri = M.read_best(path)
+    m, (v1, v2, v3, ta, tv) = _prepared(oracle, "2eu7_02562")
+    n = m["nar"]
+    solver.set_mesh(tri)
+    solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+    solver.solve_mesh(precision=64)
+    st = solver.hm_stats()
+    print("hm stats", st, flush=True)
+    assert st["n_lowrank"] > 0 and st["kmax"] > 0
+    x_o, gr = oracle.tgetgr(m)
+    icomp = np.ones(n, dtype=np.int32)
+    oracle.deflate(m, gr, icomp, 1)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((3 * n, 4))
+    y = solver.hm_apply(x)
+    want = gr @ x
+    err = np.linalg.norm(y - want) / np.linalg.norm(want)
+    print(f"product relative error {err:.2e}", flush=True)
+    assert err < 10 * EPS
+    # deterministic: the same product twice, bit for bit
+    assert np.array_equal(solver.hm_apply(x), y)
+    solver.set_method("dense")
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_transport_matches_dense_oracle(bh, oracle, solver, single):
+    from best_hydro_amd import meshes as M
+    path = os.path.join(GOLDEN, "2eu7_02562")
+    v1, v2, v3, ta, tv = oracle.read_best(path)
+    ref = oracle.best_run(v1, v2, v3, ta, tv, iarea=1, cg=1.0, ivisco=1, icov=0)
+    solver.set_mesh(M.read_best(path))
+    solver.set_method("hmatrix", eps=EPS, single=single, tol=1e-10)
+    r = solver.solve_mesh(ivisco=1)
+    st = solver.hm_stats()
+    print(f"single={single}: iterations {st['iters_min']}..{st['iters_max']}, relres {st['relres']:.1e}, "
+          f"D_t {rel(r['dtrans'], ref['dtrans']):.1e} D_r {rel(r['dr'], ref['dr']):.1e} "
+          f"eta {rel(r['chiprime'], ref['chiprime']):.1e}", flush=True)
+    assert st["relres"] <= 1e-10
+    tol = 1e-5 if not single else 3e-5
+    assert rel(r["dtrans"], ref["dtrans"]) < tol
+    assert rel(r["dr"], ref["dr"]) < tol
+    assert rel(r["chiprime"], ref["chiprime"]) < tol
+    solver.set_method("dense")
+
+
+def test_two_surfaces(bh, oracle, solver):
+    """NDEF = 2: the per-surface deflation in the H-matrix product."""
+    from best_hydro_amd import meshes as M
+    path = os.path.join(GOLDEN, "1aki49_atom_01806")
+    v1, v2, v3, ta, tv = oracle.read_best(path)
+    ref = oracle.best_run(v1, v2, v3, ta, tv, iarea=1, cg=1.0, ivisco=1, icov=0)
+    solver.set_mesh(M.read_best(path))
+    solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+    r = solver.solve_mesh(ivisco=1)
+    assert r["ndef"] == 2
+    assert rel(r["dtrans"], ref["dtrans"]) < 1e-5 and rel(r["chiprime"], ref["chiprime"]) < 1e-5
+    solver.set_method("dense")
+
+
+def test_hmatrix_abi_against_reference_library(bh, oracle, monkeypatch):
+    """bhm_build/bhm_solve on the H-matrix path (BEST_HIP_SOLVER=hmatrix) and the reference's
+    library at the same eps: both close to LAPACK on the dense GR, and to each other."""
+    R = oracle.reference_lib()
+    m, _ = _prepared(oracle, "2eu7_02562")
+    n = m["nar"]
+    icomp, ncomp, nopen = oracle.compon(m)
+    x_o, gr = oracle.tgetgr(m)
+    oracle.deflate(m, gr, icomp, ncomp)
+    lu, piv, info = oracle.lapack_getrf(gr)
+    xo, sb = oracle.self_blocks(m)
+    b = oracle.difco_rhs(n, xo)
+    want = oracle.lapack_getrs(lu, piv, b.copy())
+    monkeypatch.setenv("BEST_HIP_SOLVER", "hmatrix")
+    monkeypatch.setenv("BEST_HMATRIX_GMRES_TOL", "1e-10")
+    hm = bh.hm
+    assert hm.create(n, m["v1"], m["v2"], m["v3"], m["area"], m["afac"], 1.0, 0) == 0
+    diagm = hm.diag_mean()
+    xmu, gdef = oracle.deflation_weights(m, diagm, icomp, ncomp)
+    assert hm.build(EPS, 0, 0.1, ncomp, gdef, icomp, xmu, 0, 6) == 0
+    rc, sol = hm.solve(b.copy())
+    lines = hm.log_lines()
+    print("\n".join(lines), flush=True)
+    assert rc == 0
+    assert any("GMRES" in l for l in lines)
+    e_gpu = np.max(np.abs(sol - want)) / np.max(np.abs(want))
+    print(f"GPU H-matrix vs LAPACK: {e_gpu:.2e}", flush=True)
+    assert e_gpu < 1e-4
+    hm.destroy()
+    if R is None:
+        return
+    assert R.bhm_create(n, m["v1"], m["v2"], m["v3"], m["area"], m["afac"], 1.0, 8) == 0
+    assert R.bhm_build(EPS, 0, 0.1, ncomp, gdef, icomp, xmu, 0, 6) == 0
+    bref = b.copy(order="F").reshape(-1, order="F")
+    assert R.bhm_solve(6, bref) == 0
+    sol_ref = bref.reshape(b.shape, order="F")
+    e_ref = np.max(np.abs(sol_ref - want)) / np.max(np.abs(want))
+    e_pair = np.max(np.abs(sol - sol_ref)) / np.max(np.abs(want))
+    print(f"reference H-matrix vs LAPACK: {e_ref:.2e}; GPU vs reference: {e_pair:.2e}", flush=True)
+    assert e_pair < max(2 * e_ref, 1e-4)
