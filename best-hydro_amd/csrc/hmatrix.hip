@@ -482,7 +482,8 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
    }
 }
 
-// The first k columns of U and V from the work space into the block's storage (E).
+// The first k columns of U (column-major) and of V (stored transposed, the rank index fastest)
+// from the work space into the block's storage (E).
 template <typename E>
 __global__ void k_hm_compact( const AcaJob* __restrict__ jobs, const int* __restrict__ rank, void* const* __restrict__ uout,
                               void* const* __restrict__ vout ) {
@@ -493,7 +494,11 @@ __global__ void k_hm_compact( const AcaJob* __restrict__ jobs, const int* __rest
    E* Uo = static_cast<E*>( uout[ blockIdx.x ] );
    E* Vo = static_cast<E*>( vout[ blockIdx.x ] );
    for ( long e = threadIdx.x; e < M * k; e += blockDim.x ) Uo[ e ] = (E)J.U[ e ];
-   for ( long e = threadIdx.x; e < N * k; e += blockDim.x ) Vo[ e ] = (E)J.V[ e ];
+   // V with the rank index fastest: Vo[c k + l] = V(c, l)
+   for ( long e = threadIdx.x; e < N * k; e += blockDim.x ) {
+      const long c = e / k, l = e % k;
+      Vo[ e ] = (E)J.V[ l * N + c ];
+   }
 }
 
 // ------------------------------------------------------------------ the product
@@ -503,133 +508,176 @@ struct BlkDev {
    int nr, nc;
    long c0;
    const void* a;  // dense: D (nr x nc); low-rank: U (nr x rank)
-   const void* b;  // low-rank: V (nc x rank)
+   const void* b;  // low-rank: V (nc x rank), stored with the rank index fastest
    long w;         // low-rank: offset of V^T X in the work array (rank x p)
 };
 
-// W_b = V_b^T X for every low-rank block: one workgroup per block; each thread sums its columns
-// for 4 ranks and 8 right-hand sides at a time, then a fixed-order reduction.
-template <typename E>
-__global__ void __launch_bounds__( kWg ) k_hm_vtx( const BlkDev* __restrict__ blks, const int* __restrict__ lr, int p,
-                                                   const double* __restrict__ X, long ldx, double* __restrict__ W ) {
-   __shared__ double red[ 4 ][ 32 ];
-   const BlkDev B = blks[ lr[ blockIdx.x ] ];
-   const E* V = static_cast<const E*>( B.b );
-   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+constexpr int kSeg = 1024;  // columns of V per task of k_hm_vtx
+constexpr int kDotChunkDefl = 8192;  // rows per partial sum of the deflation product
+constexpr int kBatch = 8;   // loads in flight per lane in the product kernels
+
+// V_b^T X for every low-rank block.  V is stored with the rank index fastest (V[c * rank + l]),
+// so a wave whose lanes are 64 consecutive l reads one contiguous line per column c, and the
+// entries of X it needs, staged in LDS, are the same for every lane.  One wave per task (block,
+// l0, segment of kSeg columns); each lane keeps its P sums in registers and writes them for
+// k_hm_wsum.
+template <typename E, int P>
+__global__ void __launch_bounds__( 64 ) k_hm_vtx( const int4* __restrict__ tasks, const BlkDev* __restrict__ blks,
+                                                  const double* __restrict__ X, long ldx, double* __restrict__ part ) {
+   __shared__ double xs[ 256 * P ];
+   const int t = blockIdx.x, lane = threadIdx.x;
+   const int4 T = tasks[ t ];
+   const BlkDev B = blks[ T.x ];
+   const int l = T.y + lane;
+   const bool act = l < B.rank;
+   const E* v = static_cast<const E*>( B.b ) + l;
    const double* x = X + B.c0;
-   for ( int q0 = 0; q0 < p; q0 += 8 ) {
-      const int qn = min( 8, p - q0 );
-      for ( int l0 = 0; l0 < B.rank; l0 += 4 ) {
-         double acc[ 4 ][ 8 ];
+   const int lo = T.z * kSeg, hi = min( B.nc, lo + kSeg );
+   double acc[ P ];
 #pragma unroll
-         for ( int a = 0; a < 4; ++a )
-#pragma unroll
-            for ( int q = 0; q < 8; ++q ) acc[ a ][ q ] = 0.0;
-         for ( int c = tid; c < B.nc; c += kWg ) {
-            double xv[ 8 ];
-#pragma unroll
-            for ( int q = 0; q < 8; ++q ) xv[ q ] = q < qn ? x[ ( q0 + q ) * ldx + c ] : 0.0;
-#pragma unroll
-            for ( int a = 0; a < 4; ++a ) {
-               const double vv = l0 + a < B.rank ? (double)V[ (long)( l0 + a ) * B.nc + c ] : 0.0;
-#pragma unroll
-               for ( int q = 0; q < 8; ++q ) acc[ a ][ q ] += vv * xv[ q ];
-            }
-         }
-#pragma unroll
-         for ( int a = 0; a < 4; ++a )
-#pragma unroll
-            for ( int q = 0; q < 8; ++q ) {
-               const double s = wave_sum( acc[ a ][ q ] );
-               if ( lane == 0 ) red[ wave ][ 8 * a + q ] = s;
-            }
-         __syncthreads();
-         if ( tid < 32 ) {
-            const int a = tid >> 3, q = tid & 7;
-            if ( l0 + a < B.rank && q < qn )
-               W[ B.w + (long)( l0 + a ) * p + q0 + q ] = red[ 0 ][ tid ] + red[ 1 ][ tid ] + red[ 2 ][ tid ] + red[ 3 ][ tid ];
-         }
-         __syncthreads();
+   for ( int q = 0; q < P; ++q ) acc[ q ] = 0.0;
+   for ( int cb = lo; cb < hi; cb += 256 ) {
+      const int cn = min( 256, hi - cb ), cn8 = ( cn + kBatch - 1 ) / kBatch * kBatch;
+      __syncthreads();
+      for ( int i = lane; i < cn8 * P; i += 64 ) {
+         const int c = i % cn8, q = i / cn8;
+         xs[ c * P + q ] = c < cn ? x[ q * ldx + cb + c ] : 0.0;
       }
+      __syncthreads();
+      for ( int c0 = 0; c0 < cn8; c0 += kBatch ) {
+         double vv[ kBatch ];
+#pragma unroll
+         for ( int j = 0; j < kBatch; ++j ) vv[ j ] = ( act && c0 + j < cn ) ? (double)v[ (long)( cb + c0 + j ) * B.rank ] : 0.0;
+#pragma unroll
+         for ( int j = 0; j < kBatch; ++j )
+#pragma unroll
+            for ( int q = 0; q < P; ++q ) acc[ q ] = fma( vv[ j ], xs[ ( c0 + j ) * P + q ], acc[ q ] );
+      }
+   }
+#pragma unroll
+   for ( int q = 0; q < P; ++q ) part[ ( (long)t * 64 + lane ) * kMaxRhs + q ] = acc[ q ];
+}
+
+// W_b(l, q) = sum over the segments, in order, of the partial sums; one thread per (block, l).
+// wt = (block, l, task of the first segment, segments); the segments' tasks are ceil(rank / 64)
+// apart.
+__global__ void k_hm_wsum( const int4* __restrict__ wt, int nwt, const BlkDev* __restrict__ blks, int p,
+                           const double* __restrict__ part, double* __restrict__ W ) {
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if ( t >= nwt ) return;
+   const int4 T = wt[ t ];
+   const BlkDev B = blks[ T.x ];
+   const int nl0 = ( B.rank + 63 ) / 64, lane = T.y & 63;
+   for ( int q = 0; q < p; ++q ) {
+      double a = 0.0;
+      for ( int g = 0; g < T.w; ++g ) a += part[ ( ( (long)T.z + (long)g * nl0 ) * 64 + lane ) * kMaxRhs + q ];
+      W[ B.w + (long)T.y * p + q ] = a;
    }
 }
 
-// Y(rows of leaf L) = sum over the blocks covering those rows, in a fixed order, of D x or U w.
-// One workgroup per leaf cluster; thread r owns row r for every right-hand side.
-template <typename E>
-__global__ void __launch_bounds__( kRowsWg ) k_hm_rows( const int* __restrict__ leaf_r0, const int* __restrict__ leaf_nr,
-                                                        const int* __restrict__ ptr, const int* __restrict__ ent_blk,
-                                                        const int* __restrict__ ent_off, const BlkDev* __restrict__ blks,
-                                                        int p, const double* __restrict__ X, long ldx,
-                                                        const double* __restrict__ W, double* __restrict__ Y, long ldy ) {
-   __shared__ double st[ 3 * 64 * kMaxRhs ];
-   const int L = blockIdx.x, r = threadIdx.x;
+constexpr int kSplit = 8;  // waves per leaf cluster in k_hm_rows, each summing a part of its blocks
+
+// Partial Y(rows of leaf L) over part s of the blocks covering those rows (the blocks in a fixed
+// order, cut into kSplit consecutive parts): sums of D x or U w.  One wave per (leaf, part);
+// lane r owns rows r and r + 64 for every right-hand side; kBatch columns of D or U are loaded
+// before they are used, and the entries of X or W a block needs are staged in LDS (the same for
+// every lane).  k_hm_ysum adds the parts in order.
+template <typename E, int P>
+__global__ void __launch_bounds__( 64 ) k_hm_rows( const int* __restrict__ leaf_nr, const int* __restrict__ ptr,
+                                                   const int* __restrict__ ent_blk, const int* __restrict__ ent_off,
+                                                   const BlkDev* __restrict__ blks, const double* __restrict__ X, long ldx,
+                                                   const double* __restrict__ W, double* __restrict__ ypart ) {
+   __shared__ double st[ 256 * P ];
+   const int L = blockIdx.x, part = blockIdx.y, r = threadIdx.x;
    const int nr = leaf_nr[ L ];
-   double acc[ kMaxRhs ];
+   const bool act0 = r < nr, act1 = r + 64 < nr;
+   const int e0 = ptr[ L ], cnt = ptr[ L + 1 ] - e0;
+   const int elo = e0 + (int)( (long)cnt * part / kSplit ), ehi = e0 + (int)( (long)cnt * ( part + 1 ) / kSplit );
+   double acc0[ P ], acc1[ P ];
 #pragma unroll
-   for ( int q = 0; q < kMaxRhs; ++q ) acc[ q ] = 0.0;
-   for ( int e = ptr[ L ]; e < ptr[ L + 1 ]; ++e ) {
+   for ( int q = 0; q < P; ++q ) acc0[ q ] = acc1[ q ] = 0.0;
+   for ( int e = elo; e < ehi; ++e ) {
       const BlkDev B = blks[ ent_blk[ e ] ];
-      const long off = ent_off[ e ];
-      const E* A = static_cast<const E*>( B.a );
-      if ( B.type == kDense ) {
-         for ( int c0 = 0; c0 < B.nc; c0 += 3 * 64 ) {
-            const int cn = min( 3 * 64, B.nc - c0 );
-            __syncthreads();
-            for ( int t = r; t < cn * p; t += kRowsWg ) {
-               const int c = t % cn, q = t / cn;
-               st[ c * kMaxRhs + q ] = X[ q * ldx + B.c0 + c0 + c ];
+      const E* A = static_cast<const E*>( B.a ) + ent_off[ e ] + r;
+      const bool dense = B.type == kDense;
+      const int ncol = dense ? B.nc : B.rank;
+      for ( int cb = 0; cb < ncol; cb += 256 ) {
+         const int cn = min( 256, ncol - cb ), cn8 = ( cn + kBatch - 1 ) / kBatch * kBatch;
+         __syncthreads();
+         if ( dense ) {
+            const double* x = X + B.c0 + cb;
+            for ( int i = r; i < cn8 * P; i += 64 ) {
+               const int c = i % cn8, q = i / cn8;
+               st[ c * P + q ] = c < cn ? x[ q * ldx + c ] : 0.0;
             }
-            __syncthreads();
-            if ( r < nr ) {
-               for ( int c = 0; c < cn; ++c ) {
-                  const double d = (double)A[ (long)( c0 + c ) * B.nr + off + r ];
-#pragma unroll
-                  for ( int q = 0; q < kMaxRhs; ++q )
-                     if ( q < p ) acc[ q ] += d * st[ c * kMaxRhs + q ];
-               }
-            }
+         } else {
+            const double* w = W + B.w + (long)cb * P;
+            for ( int i = r; i < cn8 * P; i += 64 ) st[ i ] = i < cn * P ? w[ i ] : 0.0;
          }
-      } else {
-         for ( int l0 = 0; l0 < B.rank; l0 += 3 * 64 ) {
-            const int ln = min( 3 * 64, B.rank - l0 );
-            __syncthreads();
-            for ( int t = r; t < ln * p; t += kRowsWg ) st[ ( t / p ) * kMaxRhs + t % p ] = W[ B.w + (long)l0 * p + t ];
-            __syncthreads();
-            if ( r < nr ) {
-               for ( int l = 0; l < ln; ++l ) {
-                  const double uu = (double)A[ (long)( l0 + l ) * B.nr + off + r ];
+         __syncthreads();
+         for ( int c0 = 0; c0 < cn8; c0 += kBatch ) {
+            double a0[ kBatch ], a1[ kBatch ];
 #pragma unroll
-                  for ( int q = 0; q < kMaxRhs; ++q )
-                     if ( q < p ) acc[ q ] += uu * st[ l * kMaxRhs + q ];
-               }
+            for ( int j = 0; j < kBatch; ++j ) {
+               const bool in = c0 + j < cn;
+               const long o = (long)( cb + c0 + j ) * B.nr;
+               a0[ j ] = ( act0 && in ) ? (double)A[ o ] : 0.0;
+               a1[ j ] = ( act1 && in ) ? (double)A[ o + 64 ] : 0.0;
             }
+#pragma unroll
+            for ( int j = 0; j < kBatch; ++j )
+#pragma unroll
+               for ( int q = 0; q < P; ++q ) {
+                  const double xv = st[ ( c0 + j ) * P + q ];
+                  acc0[ q ] = fma( a0[ j ], xv, acc0[ q ] );
+                  acc1[ q ] = fma( a1[ j ], xv, acc1[ q ] );
+               }
          }
       }
    }
-   if ( r < nr ) {
-      const long row = leaf_r0[ L ] + r;
+   double* out = ypart + ( (long)L * kSplit + part ) * kRowsWg * kMaxRhs;
 #pragma unroll
-      for ( int q = 0; q < kMaxRhs; ++q )
-         if ( q < p ) Y[ q * ldy + row ] = acc[ q ];
+   for ( int q = 0; q < P; ++q ) {
+      if ( act0 ) out[ q * kRowsWg + r ] = acc0[ q ];
+      if ( act1 ) out[ q * kRowsWg + r + 64 ] = acc1[ q ];
    }
 }
 
-// s[k][q] = sum over the unknowns of surface k of xmu x(:, q), one workgroup of 1024 threads,
-// fixed-order sums; then y += gdef[comp] xmu s[comp].
-__global__ void __launch_bounds__( 1024 ) k_hm_defl_dot( int n3, int ndef, int p, const double* __restrict__ xmu,
+// Y(row, q) = the kSplit parts in order; one thread per (leaf, row slot, q).
+__global__ void k_hm_ysum( int nleaves, int p, const int* __restrict__ leaf_r0, const int* __restrict__ leaf_nr,
+                           const double* __restrict__ ypart, double* __restrict__ Y, long ldy ) {
+   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if ( t >= (long)nleaves * p * kRowsWg ) return;
+   const int r = (int)( t % kRowsWg ), q = (int)( ( t / kRowsWg ) % p ), L = (int)( t / ( (long)kRowsWg * p ) );
+   if ( r >= leaf_nr[ L ] ) return;
+   const double* in = ypart + (long)L * kSplit * kRowsWg * kMaxRhs + q * kRowsWg + r;
+   double a = 0.0;
+#pragma unroll
+   for ( int s = 0; s < kSplit; ++s ) a += in[ (long)s * kRowsWg * kMaxRhs ];
+   Y[ q * ldy + leaf_r0[ L ] + r ] = a;
+}
+
+__global__ void k_hm_defl_final( int count, int nchunk, const double* __restrict__ part, double* __restrict__ out ) {
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if ( t >= count ) return;
+   double a = 0.0;
+   for ( int c = 0; c < nchunk; ++c ) a += part[ (long)t * nchunk + c ];
+   out[ t ] = a;
+}
+
+// Partial sums of xmu x(:, q) over the unknowns of surface k (all unknowns when ndef = 1), one
+// workgroup per (chunk of rows, k, q); k_hm_dots_final adds the chunks in order into s[k p + q].
+__global__ void __launch_bounds__( kWg ) k_hm_defl_part( int n3, int ndef, int p, const double* __restrict__ xmu,
                                                          const int* __restrict__ comp, const double* __restrict__ X,
-                                                         long ldx, double* __restrict__ s ) {
+                                                         long ldx, double* __restrict__ part ) {
    __shared__ double red[ 16 ];
-   for ( int kk = 0; kk < ndef; ++kk )
-      for ( int q = 0; q < p; ++q ) {
-         double a = 0.0;
-         for ( int i = threadIdx.x; i < n3; i += blockDim.x )
-            if ( ndef == 1 || comp[ i / 3 ] == kk + 1 ) a += xmu[ i ] * X[ q * ldx + i ];
-         a = wg_sum( a, red );
-         if ( threadIdx.x == 0 ) s[ kk * kMaxRhs + q ] = a;
-         __syncthreads();
-      }
+   const int c = blockIdx.x, nchunk = gridDim.x, kq = blockIdx.y, kk = kq / p, q = kq % p;
+   const long lo = (long)c * kDotChunkDefl, hi = min( (long)n3, lo + kDotChunkDefl );
+   double a = 0.0;
+   for ( long i = lo + threadIdx.x; i < hi; i += kWg )
+      if ( ndef == 1 || comp[ i / 3 ] == kk + 1 ) a += xmu[ i ] * X[ q * ldx + i ];
+   a = wg_sum( a, red );
+   if ( threadIdx.x == 0 ) part[ (long)kq * nchunk + c ] = a;
 }
 
 __global__ void k_hm_defl_add( int n3, int ndef, int p, const double* __restrict__ xmu, const int* __restrict__ comp,
@@ -639,7 +687,7 @@ __global__ void k_hm_defl_add( int n3, int ndef, int p, const double* __restrict
    if ( t >= (long)n3 * p ) return;
    const int i = (int)( t % n3 ), q = (int)( t / n3 );
    const int kk = ndef == 1 ? 0 : comp[ i / 3 ] - 1;
-   Y[ q * ldy + i ] += gdef[ kk ] * xmu[ i ] * s[ kk * kMaxRhs + q ];
+   Y[ q * ldy + i ] += gdef[ kk ] * xmu[ i ] * s[ kk * p + q ];
 }
 
 // ------------------------------------------------------------------ block Jacobi preconditioner
@@ -823,19 +871,22 @@ struct HMatrix {
    // block storage
    std::deque<DevBuf<char>> pools;  // a deque: growing it never moves (and frees) an allocation
    DevBuf<BlkDev> blk_dev;
-   DevBuf<int> lr_list;           // indices of the low-rank blocks
+   DevBuf<int4> vtx_tasks, w_tasks;  // (block, l, segment) and (block, l, first task, segments)
+   DevBuf<double> vtx_part;
+   int n_vtx = 0, n_w = 0;
    int n_lr = 0;
    long w_elems = 0;              // rank * p summed (p = kMaxRhs)
    DevBuf<double> w;              // V^T X per low-rank block
    // leaf row lists
    DevBuf<int> leaf_r0, leaf_nr, ptr, ent_blk, ent_off;
+   DevBuf<double> ypart;          // kSplit partial sums per leaf row
    int nleaves = 0;
    // preconditioner
    DevBuf<int> diag_blk;
    DevBuf<long> pinv_off;
    DevBuf<double> pinv;
    // deflation sums and Krylov work space
-   DevBuf<double> defl_s;
+   DevBuf<double> defl_s, defl_part;
    DevBuf<double> Vb, Wk, Zk, Xk, Bk, Rk, part, hbuf;
 
    void* alloc( size_t bytes ) {
@@ -843,21 +894,44 @@ struct HMatrix {
       return pools.back().ensure( std::max<size_t>( bytes, 16 ) );
    }
 
+   template <typename E, int P>
+   void product( hipStream_t st, const double* X, double* Y ) {
+      if ( n_vtx > 0 ) {
+         hipLaunchKernelGGL( ( k_hm_vtx<E, P> ), dim3( n_vtx ), dim3( 64 ), 0, st, vtx_tasks.p, blk_dev.p, X, (long)n3,
+                             vtx_part.p );
+         hipLaunchKernelGGL( k_hm_wsum, dim3( nblocks( n_w, 256 ) ), dim3( 256 ), 0, st, w_tasks.p, n_w, blk_dev.p, P, vtx_part.p,
+                             w.p );
+      }
+      hipLaunchKernelGGL( ( k_hm_rows<E, P> ), dim3( nleaves, kSplit ), dim3( 64 ), 0, st, leaf_nr.p, ptr.p, ent_blk.p, ent_off.p,
+                          blk_dev.p, X, (long)n3, w.p, ypart.p );
+   }
+
+   template <typename E>
+   void product_p( hipStream_t st, int p, const double* X, double* Y ) {
+      switch ( p ) {
+#define BH_HM_CASE( P ) \
+   case P: product<E, P>( st, X, Y ); break;
+         BH_HM_CASE( 1 ) BH_HM_CASE( 2 ) BH_HM_CASE( 3 ) BH_HM_CASE( 4 ) BH_HM_CASE( 5 ) BH_HM_CASE( 6 ) BH_HM_CASE( 7 )
+         BH_HM_CASE( 8 ) BH_HM_CASE( 9 ) BH_HM_CASE( 10 ) BH_HM_CASE( 11 ) BH_HM_CASE( 12 ) BH_HM_CASE( 13 )
+         BH_HM_CASE( 14 ) BH_HM_CASE( 15 ) BH_HM_CASE( 16 )
+#undef BH_HM_CASE
+         default: throw std::runtime_error( "H-matrix product: 1 to 16 columns" );
+      }
+   }
+
    // Y = A X in tree order (n3 x p, leading dimension n3).
    void apply( hipStream_t st, int p, const double* X, double* Y ) {
-      if ( n_lr > 0 ) {
-         if ( single )
-            hipLaunchKernelGGL( k_hm_vtx<float>, dim3( n_lr ), dim3( kWg ), 0, st, blk_dev.p, lr_list.p, p, X, (long)n3, w.p );
-         else
-            hipLaunchKernelGGL( k_hm_vtx<double>, dim3( n_lr ), dim3( kWg ), 0, st, blk_dev.p, lr_list.p, p, X, (long)n3, w.p );
-      }
       if ( single )
-         hipLaunchKernelGGL( k_hm_rows<float>, dim3( nleaves ), dim3( kRowsWg ), 0, st, leaf_r0.p, leaf_nr.p, ptr.p, ent_blk.p,
-                             ent_off.p, blk_dev.p, p, X, (long)n3, w.p, Y, (long)n3 );
+         product_p<float>( st, p, X, Y );
       else
-         hipLaunchKernelGGL( k_hm_rows<double>, dim3( nleaves ), dim3( kRowsWg ), 0, st, leaf_r0.p, leaf_nr.p, ptr.p, ent_blk.p,
-                             ent_off.p, blk_dev.p, p, X, (long)n3, w.p, Y, (long)n3 );
-      hipLaunchKernelGGL( k_hm_defl_dot, dim3( 1 ), dim3( 1024 ), 0, st, n3, ndef, p, xmu_t.p, comp_t.p, X, (long)n3, defl_s.p );
+         product_p<double>( st, p, X, Y );
+      hipLaunchKernelGGL( k_hm_ysum, dim3( nblocks( (long)nleaves * p * kRowsWg, 256 ) ), dim3( 256 ), 0, st, nleaves, p,
+                          leaf_r0.p, leaf_nr.p, ypart.p, Y, (long)n3 );
+      const int nchunk = ( n3 + kDotChunkDefl - 1 ) / kDotChunkDefl;
+      hipLaunchKernelGGL( k_hm_defl_part, dim3( nchunk, ndef * p ), dim3( kWg ), 0, st, n3, ndef, p, xmu_t.p, comp_t.p, X, (long)n3,
+                          defl_part.p );
+      hipLaunchKernelGGL( k_hm_defl_final, dim3( nblocks( ndef * p, 64 ) ), dim3( 64 ), 0, st, ndef * p, nchunk, defl_part.p,
+                          defl_s.p );
       hipLaunchKernelGGL( k_hm_defl_add, dim3( nblocks( (long)n3 * p, 256 ) ), dim3( 256 ), 0, st, n3, ndef, p, xmu_t.p, comp_t.p,
                           gdef.p, defl_s.p, Y, (long)n3 );
    }
@@ -1107,8 +1181,27 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
    h.w.ensure( std::max<long>( wofs, 1 ) );
    h.blk_dev.ensure( bd.size() );
    BH_HIP( hipMemcpyAsync( h.blk_dev.p, bd.data(), sizeof( BlkDev ) * bd.size(), hipMemcpyHostToDevice, st ) );
-   h.lr_list.ensure( std::max<size_t>( lrl.size(), 1 ) );
-   BH_HIP( hipMemcpyAsync( h.lr_list.p, lrl.data(), sizeof( int ) * lrl.size(), hipMemcpyHostToDevice, st ) );
+   {
+      // tasks of the V^T X product, blocks in the order of the build (largest first)
+      std::vector<int4> vt, wt;
+      for ( int i : lr ) {
+         const Blk& b = h.blks[ i ];
+         if ( b.type != kLowRank || b.rank <= 0 ) continue;
+         const int nseg = (int)( ( b.nc + kSeg - 1 ) / kSeg ), nl0 = ( b.rank + 63 ) / 64;
+         const int first = (int)vt.size();
+         for ( int g = 0; g < nseg; ++g )
+            for ( int l0 = 0; l0 < nl0; ++l0 ) vt.push_back( make_int4( i, 64 * l0, g, 0 ) );
+         for ( int l = 0; l < b.rank; ++l ) wt.push_back( make_int4( i, l, first + l / 64, nseg ) );
+      }
+      h.n_vtx = (int)vt.size();
+      h.n_w = (int)wt.size();
+      h.vtx_tasks.ensure( std::max<size_t>( vt.size(), 1 ) );
+      h.w_tasks.ensure( std::max<size_t>( wt.size(), 1 ) );
+      h.vtx_part.ensure( std::max<size_t>( vt.size(), 1 ) * 64 * kMaxRhs );
+      BH_HIP( hipMemcpyAsync( h.vtx_tasks.p, vt.data(), sizeof( int4 ) * vt.size(), hipMemcpyHostToDevice, st ) );
+      BH_HIP( hipMemcpyAsync( h.w_tasks.p, wt.data(), sizeof( int4 ) * wt.size(), hipMemcpyHostToDevice, st ) );
+      BH_HIP( hipStreamSynchronize( st ) );
+   }
 
    // per leaf cluster: the blocks covering its rows, in block order (fixed)
    h.nleaves = (int)T.leaves.size();
@@ -1185,6 +1278,8 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
    S.t_pre = now() - t0;
    say( "preconditioner: block Jacobi on the %d diagonal leaf blocks, %.3f s", h.nleaves, S.t_pre );
    h.defl_s.ensure( (size_t)h.ndef * kMaxRhs );
+   h.defl_part.ensure( (size_t)( ( h.n3 + kDotChunkDefl - 1 ) / kDotChunkDefl ) * h.ndef * kMaxRhs );
+   h.ypart.ensure( (size_t)h.nleaves * kSplit * kRowsWg * kMaxRhs );
    if ( stats ) *stats = S;
    return H;
 }
