@@ -136,6 +136,8 @@ def lib():
     L.bhg_besolv.restype = i
     L.bhg_set_method.argtypes = [vp, i, d, i, d]
     L.bhg_set_method.restype = i
+    L.bhg_set_hm_option.argtypes = [vp, C.c_char_p, d]
+    L.bhg_set_hm_option.restype = i
     L.bhg_get_hm_stats.argtypes = [vp, C.POINTER(BhHmStats)]
     L.bhg_get_hm_stats.restype = i
     L.bhg_hm_plan.argtypes = [i, _dp, _dp, _dp, _dp, i, d, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
@@ -240,6 +242,11 @@ class Solver:
         """'dense' (LU) or 'hmatrix' (cross-approximated blocks + GMRES) for the next solve_mesh."""
         m = {"dense": 0, "hmatrix": 1}[method]
         self._check(self.L.bhg_set_method(self.h, m, float(eps), int(bool(single)), float(tol)), "set_method")
+
+    def set_hm_option(self, name, value):
+        """One of the H-matrix settings that are not BEST options (include/best_hip.h)."""
+        if self.L.bhg_set_hm_option(self.h, name.encode(), float(value)) != 0:
+            raise HipError(f"unknown H-matrix option {name!r}")
 
     def hm_stats(self):
         s = BhHmStats()

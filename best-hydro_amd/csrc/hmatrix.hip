@@ -320,8 +320,12 @@ struct AcaJob {
 // step evaluates the three rows of one row triangle, takes up to three rank-one terms from them,
 // each with the column of its pivot; it stops after a row step whose every new term had
 // |u| |v| <= eps |S|_F, S the approximation so far.  rank[job] = the rank, or -1 at kcap.
-__global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jobs, KernelData K, double eps,
-                                                   int* __restrict__ rank, unsigned long long* __restrict__ calls ) {
+// One workgroup of NT threads per block (512 for the large blocks, whose products with the
+// terms so far stream O(k (M + N)) entries per step); those products load 4 terms at a time.
+template <int NT>
+__global__ void __launch_bounds__( NT ) k_hm_aca( const AcaJob* __restrict__ jobs, KernelData K, double eps,
+                                                  int* __restrict__ rank, unsigned long long* __restrict__ calls ) {
+   constexpr int NW = NT / 64;
    __shared__ double red[ 16 ];
    __shared__ long ired[ 16 ];
    __shared__ double dots[ 2 * 256 ];
@@ -334,7 +338,7 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
    double* R = J.R;
    double* u = J.u;
    double* v = J.v;
-   for ( int it = tid; it < m; it += kWg ) J.used[ it ] = 0;
+   for ( int it = tid; it < m; it += NT ) J.used[ it ] = 0;
    unsigned long long ncalls = 0;
    int k = 0, zero_rows = 0;
    double norm2 = 0.0, piv_max = 0.0;
@@ -342,8 +346,8 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
    int result = -2;
    __syncthreads();
    for ( ;; ) {
-      // the residual rows of row triangle i
-      for ( int jt = tid; jt < n; jt += kWg ) {
+      // the rows of row triangle i, less the terms so far
+      for ( int jt = tid; jt < n; jt += NT ) {
          double g[ 9 ];
          kblock( K, J.rb + i, J.cb + jt, g );
 #pragma unroll
@@ -353,16 +357,29 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
       }
       ncalls += n;
       __syncthreads();
-      for ( long j = tid; j < N; j += kWg ) {
+      for ( long j = tid; j < N; j += NT ) {
+         double r0 = R[ j ], r1 = R[ N + j ], r2 = R[ 2 * N + j ];
+         int l = 0;
+         for ( ; l + 4 <= k; l += 4 ) {
+            double vv[ 4 ];
 #pragma unroll
-         for ( int a = 0; a < 3; ++a ) {
-            double r = R[ a * N + j ];
-            for ( int l = 0; l < k; ++l ) {
-               const double ua = U[ l * M + 3 * i + a ];
-               if ( ua != 0.0 ) r -= ua * V[ l * N + j ];
+            for ( int d = 0; d < 4; ++d ) vv[ d ] = V[ ( l + d ) * N + j ];
+#pragma unroll
+            for ( int d = 0; d < 4; ++d ) {
+               const double* ul = U + ( l + d ) * M + 3L * i;
+               r0 -= ul[ 0 ] * vv[ d ];
+               r1 -= ul[ 1 ] * vv[ d ];
+               r2 -= ul[ 2 ] * vv[ d ];
             }
-            R[ a * N + j ] = r;
          }
+         for ( ; l < k; ++l ) {
+            const double vv = V[ l * N + j ];
+            const double* ul = U + l * M + 3L * i;
+            r0 -= ul[ 0 ] * vv;
+            r1 -= ul[ 1 ] * vv;
+            r2 -= ul[ 2 ] * vv;
+         }
+         R[ j ] = r0, R[ N + j ] = r1, R[ 2 * N + j ] = r2;
       }
       if ( tid == 0 ) J.used[ i ] = 1;
       __syncthreads();
@@ -373,7 +390,7 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
          double best = 0.0;
          long bi = -1;
          for ( int a = 0; a < 3; ++a )
-            for ( long j = tid; j < N; j += kWg ) {
+            for ( long j = tid; j < N; j += NT ) {
                const double w = fabs( R[ a * N + j ] );
                if ( w > best ) best = w, bi = a * N + j;
             }
@@ -385,8 +402,8 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
          const long jb = bi % N;
          const int jt = (int)( jb / 3 ), bc = (int)( jb % 3 );
          const double piv = R[ ab * N + jb ];
-         // the pivot column
-         for ( int it = tid; it < m; it += kWg ) {
+         // the pivot column, less the terms so far
+         for ( int it = tid; it < m; it += NT ) {
             double g[ 9 ];
             kblock( K, J.rb + it, J.cb + jt, g );
 #pragma unroll
@@ -394,19 +411,24 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
          }
          ncalls += m;
          __syncthreads();
-         for ( long r = tid; r < M; r += kWg ) {
+         for ( long r = tid; r < M; r += NT ) {
             double w = u[ r ];
-            for ( int l = 0; l < k; ++l ) {
-               const double vjl = V[ l * N + jb ];
-               if ( vjl != 0.0 ) w -= vjl * U[ l * M + r ];
+            int l = 0;
+            for ( ; l + 4 <= k; l += 4 ) {
+               double uu[ 4 ];
+#pragma unroll
+               for ( int d = 0; d < 4; ++d ) uu[ d ] = U[ ( l + d ) * M + r ];
+#pragma unroll
+               for ( int d = 0; d < 4; ++d ) w -= V[ ( l + d ) * N + jb ] * uu[ d ];
             }
+            for ( ; l < k; ++l ) w -= V[ l * N + jb ] * U[ l * M + r ];
             u[ r ] = w / piv;
          }
-         for ( long j = tid; j < N; j += kWg ) v[ j ] = R[ ab * N + j ];
+         for ( long j = tid; j < N; j += NT ) v[ j ] = R[ ab * N + j ];
          __syncthreads();
          const double ui0 = u[ 3L * i ], ui1 = u[ 3L * i + 1 ], ui2 = u[ 3L * i + 2 ];
          double nu = 0.0, nv = 0.0;
-         for ( long j = tid; j < N; j += kWg ) {
+         for ( long j = tid; j < N; j += NT ) {
             const double vj = v[ j ];
             R[ j ] -= ui0 * vj;
             R[ N + j ] -= ui1 * vj;
@@ -414,7 +436,7 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
             nv += vj * vj;
             V[ k * N + j ] = vj;
          }
-         for ( long r = tid; r < M; r += kWg ) {
+         for ( long r = tid; r < M; r += NT ) {
             const double ur = u[ r ];
             nu += ur * ur;
             U[ k * M + r ] = ur;
@@ -423,13 +445,23 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
          nv = wg_sum( nv, red );
          // cross terms: sum_l (U_l . u)(V_l . v), one wave per l
          const int wave = tid >> 6, lane = tid & 63;
-         for ( int l = wave; l < k; l += kWg / 64 ) {
-            double du = 0.0, dv = 0.0;
-            for ( long r = lane; r < M; r += 64 ) du += U[ l * M + r ] * u[ r ];
-            for ( long j = lane; j < N; j += 64 ) dv += V[ l * N + j ] * v[ j ];
-            du = wave_sum( du );
-            dv = wave_sum( dv );
-            if ( lane == 0 && l < 256 ) dots[ 2 * l ] = du, dots[ 2 * l + 1 ] = dv;
+         for ( int l = wave; l < k; l += NW ) {
+            const double* Ul = U + l * M;
+            const double* Vl = V + l * N;
+            double du[ 4 ] = { 0, 0, 0, 0 }, dv[ 4 ] = { 0, 0, 0, 0 };
+            long r = lane;
+            for ( ; r + 192 < M; r += 256 )
+#pragma unroll
+               for ( int d = 0; d < 4; ++d ) du[ d ] += Ul[ r + 64 * d ] * u[ r + 64 * d ];
+            for ( ; r < M; r += 64 ) du[ 0 ] += Ul[ r ] * u[ r ];
+            long j = lane;
+            for ( ; j + 192 < N; j += 256 )
+#pragma unroll
+               for ( int d = 0; d < 4; ++d ) dv[ d ] += Vl[ j + 64 * d ] * v[ j + 64 * d ];
+            for ( ; j < N; j += 64 ) dv[ 0 ] += Vl[ j ] * v[ j ];
+            const double su = wave_sum( ( du[ 0 ] + du[ 1 ] ) + ( du[ 2 ] + du[ 3 ] ) );
+            const double sv = wave_sum( ( dv[ 0 ] + dv[ 1 ] ) + ( dv[ 2 ] + dv[ 3 ] ) );
+            if ( lane == 0 ) dots[ 2 * l ] = su, dots[ 2 * l + 1 ] = sv;
          }
          __syncthreads();
          double cross = 0.0;
@@ -451,7 +483,7 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
       double bu = 0.0;
       long nx = -1;
       if ( k > 0 ) {
-         for ( int it = tid; it < m; it += kWg ) {
+         for ( int it = tid; it < m; it += NT ) {
             if ( J.used[ it ] ) continue;
             const double* ul = U + ( k - 1 ) * M + 3L * it;
             const double w = fmax( fabs( ul[ 0 ] ), fmax( fabs( ul[ 1 ] ), fabs( ul[ 2 ] ) ) );
@@ -461,15 +493,15 @@ __global__ void __launch_bounds__( kWg ) k_hm_aca( const AcaJob* __restrict__ jo
       if ( bu == 0.0 ) nx = -1;
       wg_argmax( bu, nx, red, ired );
       if ( nx < 0 ) {
-         double one = 0.0;
+         double zero = 0.0;
          long first = -1;
-         for ( int it = tid; it < m; it += kWg )
+         for ( int it = tid; it < m; it += NT )
             if ( !J.used[ it ] ) {
                first = it;
                break;
             }
-         // smallest unused index: argmax of a constant with ties to the smallest index
-         wg_argmax( one, first, red, ired );
+         // the smallest unused index: an arg-max of a constant, ties to the smallest index
+         wg_argmax( zero, first, red, ired );
          nx = first;
       }
       if ( nx < 0 ) break;
@@ -513,6 +545,7 @@ struct BlkDev {
 };
 
 constexpr int kSeg = 1024;  // columns of V per task of k_hm_vtx
+constexpr long kAcaBig = 6144;  // blocks with at least this many rows + columns: 512-thread approximation
 constexpr int kDotChunkDefl = 8192;  // rows per partial sum of the deflation product
 constexpr int kBatch = 8;   // loads in flight per lane in the product kernels
 
@@ -1030,6 +1063,18 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
       DevBuf<AcaJob> jobs_d;
       DevBuf<int> rank_d;
       DevBuf<void*> uo_d, vo_d;
+      struct Streams {
+         hipStream_t s = nullptr;
+         hipEvent_t e = nullptr;
+         ~Streams() {
+            if ( e ) (void)hipEventDestroy( e );
+            if ( s ) (void)hipStreamDestroy( s );
+         }
+      } aux;
+      BH_HIP( hipStreamCreateWithFlags( &aux.s, hipStreamNonBlocking ) );
+      BH_HIP( hipEventCreateWithFlags( &aux.e, hipEventDisableTiming ) );
+      hipStream_t st_big = aux.s;
+      hipEvent_t ev = aux.e;
       size_t pos = 0;
       while ( pos < lr.size() ) {
          // a batch within the work-space budget
@@ -1070,7 +1115,23 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
          jobs_d.ensure( nj );
          rank_d.ensure( nj );
          BH_HIP( hipMemcpyAsync( jobs_d.p, jobs.data(), sizeof( AcaJob ) * nj, hipMemcpyHostToDevice, st ) );
-         hipLaunchKernelGGL( k_hm_aca, dim3( nj ), dim3( kWg ), 0, st, jobs_d.p, K, opt.aca_factor * opt.eps, rank_d.p, calls.p );
+         // the large blocks (a prefix: the batch is in decreasing size) get 512 threads, on a
+         // second stream so the small blocks run beside them
+         int nbig = 0;
+         while ( nbig < nj && 3L * ( jobs[ nbig ].m + jobs[ nbig ].nc ) >= kAcaBig ) ++nbig;
+         if ( nbig > 0 ) {
+            BH_HIP( hipEventRecord( ev, st ) );
+            BH_HIP( hipStreamWaitEvent( st_big, ev, 0 ) );
+            hipLaunchKernelGGL( k_hm_aca<512>, dim3( nbig ), dim3( 512 ), 0, st_big, jobs_d.p, K, opt.aca_factor * opt.eps,
+                                rank_d.p, calls.p );
+         }
+         if ( nj > nbig )
+            hipLaunchKernelGGL( k_hm_aca<kWg>, dim3( nj - nbig ), dim3( kWg ), 0, st, jobs_d.p + nbig, K, opt.aca_factor * opt.eps,
+                                rank_d.p + nbig, calls.p );
+         if ( nbig > 0 ) {
+            BH_HIP( hipEventRecord( ev, st_big ) );
+            BH_HIP( hipStreamWaitEvent( st, ev, 0 ) );
+         }
          std::vector<int> ranks( nj );
          BH_HIP( hipMemcpyAsync( ranks.data(), rank_d.p, sizeof( int ) * nj, hipMemcpyDeviceToHost, st ) );
          BH_HIP( hipStreamSynchronize( st ) );

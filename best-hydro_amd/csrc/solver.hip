@@ -667,6 +667,22 @@ int bhg_set_method( bhg_solver* s, int method, double eps, int single, double to
    return 0;
 }
 
+int bhg_set_hm_option( bhg_solver* s, const char* name, double value ) {
+   if ( !s || !name ) return 1;
+   const std::string k( name );
+   bhg::HmOptions& o = s->hm_opt;
+   if ( k == "eta" ) o.eta = value;
+   else if ( k == "aca_factor" ) o.aca_factor = value;
+   else if ( k == "leaf" ) o.leaf = (int)value;
+   else if ( k == "dense_small" ) o.dense_small = (long)value;
+   else if ( k == "kcap" ) o.kcap = (int)value;
+   else if ( k == "restart" ) o.restart = (int)value;
+   else if ( k == "maxsteps" ) o.maxsteps = (int)value;
+   else if ( k == "ws_gb" ) o.ws_bytes = value * 1e9;
+   else return 1;
+   return 0;
+}
+
 int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out ) {
    if ( !s || !out ) return 1;
    const bhg::HmStats& h = s->hm_stats;

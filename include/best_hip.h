@@ -162,6 +162,14 @@ typedef struct bhg_hm_stats {
 int bhg_set_method( bhg_solver* s, int method, double eps, int single, double tol );
 int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out );
 
+/* The H-matrix settings that are not BEST options (hmatrix/src/api.cpp:37-41, hmatrix.h:60-66):
+ * "eta" (admissibility, 2), "aca_factor" (0.1), "leaf" (triangles per leaf cluster, 32),
+ * "dense_small" (admissible blocks of at most this many triangle pairs stay dense, 1024), "kcap"
+ * (rank at which an approximation is abandoned for a dense block, 192), "restart" (30),
+ * "maxsteps" (500), "ws_gb" (device work space of the cross approximations, GB).  Returns 1 for
+ * an unknown name. */
+int bhg_set_hm_option( bhg_solver* s, const char* name, double value );
+
 /* The cluster and block trees alone, on the host (hmatrix/src/cluster.cpp:91-112,
  * hmatrix/src/hmatrix.cpp:27-54): out[0..6] = depth, leaf clusters, smallest and largest leaf,
  * inadmissible leaf blocks, admissible blocks above 1,024 triangle pairs, admissible blocks of at
