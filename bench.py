@@ -9,7 +9,9 @@ the DIFCO/VISCO reductions to D_t, D_r, the coupling tensor and CHI*V ([eta]).
 Prints one JSON line.  Besides the contract fields it carries the FP64-LU path's rate on the same
 mesh, the errors of both paths against the oracle (the reference's dense FP64 path restated in C
 with LAPACK), the roofline of the dominant kernel class (HIP events around every launch of the
-class during the timed steps) and the oracle's CPU rate on this host.
+class during the timed steps), the oracle's CPU rate on this host, and the H-matrix path on the
+81,920-triangle sphere (hmatrix_c5: solves/s, build and GMRES times, and the HBM roofline of the
+H-matrix product).
 """
 import argparse
 import importlib.util
@@ -58,12 +60,43 @@ def cpu_baseline(tri, cores):
     return ref, el
 
 
+def hmatrix_c5(bh, M, steps):
+    """The H-matrix path on the 81,920-triangle bumpy sphere (245,760 unknowns; dense GR would be
+    483 GB): cross approximation at eps 1e-6, GMRES on 6 right-hand sides to 1e-10.  One step is
+    a whole solve (prep, H-matrix build, GMRES, reductions) with the mesh resident."""
+    tri = M.hmatrix_sphere_mesh()
+    s = bh.Solver(0)
+    s.set_mesh(tri)
+    s.set_method("hmatrix", eps=1e-6, tol=1e-10)
+    s.solve_mesh(ivisco=0)  # warm
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = s.solve_mesh(ivisco=0)
+    el = (time.perf_counter() - t0) / steps
+    st = s.hm_stats()
+    ms, nbytes = s.hm_time_product(6, 10)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    s.close()
+    return {"workload": "40 A sphere x (1 + l<=8 bumps sd 0.05), level-6 icosphere, 81,920 triangles, 245,760 unknowns, "
+                        "eps 1e-6, 6 RHS, GMRES to 1e-10",
+            "value": round(1.0 / el, 4), "unit": "solves/s", "s_per_solve": round(el, 3), "steps": steps,
+            "build_s": round(st["t_tree"] + st["t_fill"] + st["t_pre"], 3), "gmres_s": round(st["t_solve"], 3),
+            "gmres_iterations": [st["iters_min"], st["iters_max"]], "relres": st["relres"],
+            "h_gb": round((st["bytes_dense"] + st["bytes_lowrank"]) / 1e9, 2), "rank_max": st["kmax"],
+            "rank_mean": round(st["kmean"], 1), "dtrans": r["dtrans"], "dr": r["dr"],
+            "roofline": {"kernel": "H-matrix product (k_hm_vtx + k_hm_rows + sums)", "bound": "hbm",
+                         "achieved": round(gbs, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
+                         "frac": round(gbs / PEAKS["hbm_gbs"], 4), "avg_product_ms": round(ms, 4),
+                         "bytes_per_product": nbytes}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline and error report")
+    ap.add_argument("--no-hmatrix", action="store_true", help="skip the H-matrix path's 81,920-triangle case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve does not shard across "
@@ -152,6 +185,8 @@ def main():
                                          f"OpenMP {cores} threads; LAPACK dgetrf/dgetrs via SciPy OpenBLAS; C DIFCO/VISCO): "
                                          f"{el:.2f} s"}
     s.close()
+    if not args.no_hmatrix:
+        out["hmatrix_c5"] = hmatrix_c5(bh, M, max(1, min(args.steps, 3)))
     print(json.dumps(out), flush=True)
 
 

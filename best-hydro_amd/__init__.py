@@ -142,6 +142,8 @@ def lib():
     L.bhg_get_hm_stats.restype = i
     L.bhg_hm_plan.argtypes = [i, _dp, _dp, _dp, _dp, i, d, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     L.bhg_hm_plan.restype = i
+    L.bhg_hm_time_product.argtypes = [vp, i, i, C.POINTER(d), C.POINTER(d)]
+    L.bhg_hm_time_product.restype = i
     L.bhg_hm_apply.argtypes = [vp, i, _dp, lg, _dp, lg]
     L.bhg_hm_apply.restype = i
     _LIB = L
@@ -252,6 +254,12 @@ class Solver:
         s = BhHmStats()
         self.L.bhg_get_hm_stats(self.h, C.byref(s))
         return s.as_dict()
+
+    def hm_time_product(self, nrhs=6, reps=10):
+        """(mean ms, algorithmic bytes) of the H-matrix product on device-resident vectors."""
+        ms, by = C.c_double(), C.c_double()
+        self._check(self.L.bhg_hm_time_product(self.h, int(nrhs), int(reps), C.byref(ms), C.byref(by)), "hm_time_product")
+        return ms.value, by.value
 
     def hm_apply(self, x):
         """(GR + deflation) x with the built H-matrix."""

@@ -75,6 +75,10 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
 void hm_solve( HMatrix& h, hipStream_t st, int nrhs, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
                std::vector<std::string>* log );
 
+// Mean duration (ms, HIP events) of the product with p columns of device-resident vectors over
+// reps repetitions; *bytes = the blocks' storage plus X and Y (the algorithmic HBM bytes).
+double hm_time_product( HMatrix& h, hipStream_t st, int p, int reps, double* bytes );
+
 // y = (GR + deflation) x on the device, BEST's order (for checks).
 void hm_apply( HMatrix& h, hipStream_t st, int nrhs, const double* d_x, long ldx, double* d_y, long ldy );
 

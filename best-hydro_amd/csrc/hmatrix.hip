@@ -1358,6 +1358,34 @@ void hm_apply( HMatrix& h, hipStream_t st, int p, const double* d_x, long ldx, d
    BH_HIP( hipStreamSynchronize( st ) );
 }
 
+double hm_time_product( HMatrix& h, hipStream_t st, int p, int reps, double* bytes ) {
+   if ( p < 1 || p > kMaxRhs ) throw std::runtime_error( "H-matrix product: 1 to 16 columns" );
+   const size_t nn = (size_t)h.n3 * p;
+   h.Xk.ensure( nn );
+   h.Wk.ensure( nn );
+   std::vector<double> ones( nn, 1.0 );
+   BH_HIP( hipMemcpyAsync( h.Xk.p, ones.data(), nn * sizeof( double ), hipMemcpyHostToDevice, st ) );
+   h.apply( st, p, h.Xk.p, h.Wk.p );  // warm
+   hipEvent_t e0, e1;
+   BH_HIP( hipEventCreate( &e0 ) );
+   BH_HIP( hipEventCreate( &e1 ) );
+   BH_HIP( hipEventRecord( e0, st ) );
+   for ( int r = 0; r < reps; ++r ) h.apply( st, p, h.Xk.p, h.Wk.p );
+   BH_HIP( hipEventRecord( e1, st ) );
+   BH_HIP( hipEventSynchronize( e1 ) );
+   float ms = 0.f;
+   BH_HIP( hipEventElapsedTime( &ms, e0, e1 ) );
+   (void)hipEventDestroy( e0 );
+   (void)hipEventDestroy( e1 );
+   if ( bytes ) {
+      const size_t esz = h.single ? sizeof( float ) : sizeof( double );
+      double b = 0.0;
+      for ( const Blk& k : h.blks ) b += k.type == kLowRank ? (double)( k.nr + k.nc ) * k.rank * esz : (double)k.nr * k.nc * esz;
+      *bytes = b + 2.0 * nn * sizeof( double );
+   }
+   return ms / reps;
+}
+
 void hm_solve( HMatrix& h, hipStream_t st, int p, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
                std::vector<std::string>* log ) {
    if ( p < 1 || p > kMaxRhs ) throw std::runtime_error( "H-matrix solve: 1 to 16 right-hand sides per call" );

@@ -734,6 +734,14 @@ int bhg_hm_plan( int n, const double* v1, const double* v2, const double* v3, co
    }
 }
 
+int bhg_hm_time_product( bhg_solver* s, int nrhs, int reps, double* ms, double* bytes ) {
+   return guarded( s, "bhg_hm_time_product", [&] {
+      if ( !s->hm ) throw std::runtime_error( "no H-matrix" );
+      bhg::set_device( s );
+      *ms = bhg::hm_time_product( *s->hm, s->st, nrhs, std::max( reps, 1 ), bytes );
+   } );
+}
+
 int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y, long ldy ) {
    return guarded( s, "bhg_hm_apply", [&] {
       if ( !s->hm ) throw std::runtime_error( "no H-matrix" );

@@ -177,6 +177,11 @@ int bhg_set_hm_option( bhg_solver* s, const char* name, double value );
 int bhg_hm_plan( int n, const double* v1, const double* v2, const double* v3, const double* x, int leaf, double eta,
                  long* out );
 
+/* The H-matrix product on device-resident vectors, timed with HIP events: *ms the mean over reps,
+ * *bytes the algorithmic HBM bytes of one product (the blocks' storage plus X and Y).  No
+ * counterpart in the reference: it measures the kernel hmatrix/src/hmatrix.cpp:441-471 runs. */
+int bhg_hm_time_product( bhg_solver* s, int nrhs, int reps, double* ms, double* bytes );
+
 /* y = (GR + deflation) x with the built H-matrix (hmatrix/src/hmatrix.cpp:441-471), BEST's order. */
 int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y, long ldy );
 

@@ -130,3 +130,23 @@ def test_hmatrix_abi_against_reference_library(bh, oracle, monkeypatch):
     e_pair = np.max(np.abs(sol - sol_ref)) / np.max(np.abs(want))
     print(f"reference H-matrix vs LAPACK: {e_ref:.2e}; GPU vs reference: {e_pair:.2e}", flush=True)
     assert e_pair < max(2 * e_ref, 1e-4)
+
+
+def test_large_mesh_against_gpu_dense(bh, solver):
+    """20,480 triangles (61,440 unknowns): the H-matrix answer against the dense GPU solve (FP32
+    LU + FP64 refinement, itself within 1e-10 of FP64) -- the size-independent check of the path
+    between the oracle's sizes and the 81,920-triangle case."""
+    from best_hydro_amd import meshes as M
+    tri = M.hmatrix_sphere_mesh(level=5)
+    solver.set_mesh(tri)
+    solver.set_method("dense")
+    d = solver.solve_mesh(ivisco=1, precision=32)
+    solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+    h = solver.solve_mesh(ivisco=1)
+    st = solver.hm_stats()
+    print(f"20480: iterations {st['iters_min']}..{st['iters_max']}, D_t {rel(h['dtrans'], d['dtrans']):.1e} "
+          f"D_r {rel(h['dr'], d['dr']):.1e} eta {rel(h['chiprime'], d['chiprime']):.1e}", flush=True)
+    assert st["relres"] <= 1e-10
+    for key in ("dtrans", "dr", "chiprime"):
+        assert rel(h[key], d[key]) < 1e-6, key
+    solver.set_method("dense")
