@@ -142,6 +142,25 @@ def lib():
     L.bhg_get_hm_stats.restype = i
     L.bhg_hm_plan.argtypes = [i, _dp, _dp, _dp, _dp, i, d, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     L.bhg_hm_plan.restype = i
+    L.bhg_series_create.argtypes = [i, i]
+    L.bhg_series_create.restype = vp
+    L.bhg_series_destroy.argtypes = [vp]
+    L.bhg_series_last_error.argtypes = [vp]
+    L.bhg_series_last_error.restype = C.c_char_p
+    L.bhg_series_set_mesh.argtypes = [vp, i, i, _dp, _dp, _dp, d, d]
+    L.bhg_series_set_mesh.restype = i
+    L.bhg_series_autotune.argtypes = [vp, i, d, i]
+    L.bhg_series_autotune.restype = i
+    L.bhg_series_set_tuning.argtypes = [vp, i, i, i, i]
+    L.bhg_series_set_tuning.restype = i
+    L.bhg_series_get_tuning.argtypes = [vp, i, _ip]
+    L.bhg_series_get_tuning.restype = i
+    L.bhg_series_solve.argtypes = [vp, i, d, i, i, C.POINTER(BhResult), C.POINTER(d)]
+    L.bhg_series_solve.restype = i
+    L.bhg_series_get_stats.argtypes = [vp, i, C.POINTER(BhStats)]
+    L.bhg_series_get_stats.restype = i
+    L.bhg_extrapolate_1n.argtypes = [i, _ip, _dp, C.POINTER(d), C.POINTER(d)]
+    L.bhg_extrapolate_1n.restype = i
     L.bhg_hm_time_product.argtypes = [vp, i, i, C.POINTER(d), C.POINTER(d)]
     L.bhg_hm_time_product.restype = i
     L.bhg_hm_apply.argtypes = [vp, i, _dp, lg, _dp, lg]
@@ -319,6 +338,81 @@ class Solver:
         flat = b.reshape(-1, order="F")
         self._check(self.L.bhg_besolv(self.h, nrhs, flat, 3 * self.n), "besolv")
         return flat.reshape(b.shape, order="F")
+
+
+class Series:
+    """A series of meshes of one body (``bhg_series_*``): every mesh resident, all of them solved
+    by one launch of a captured HIP graph (FP32 LU + FP64 refinement), tiles picked by the
+    autotuner; ``extrapolate`` fits each result linearly in 1/N."""
+
+    def __init__(self, meshes, device: int = 0):
+        self.L = lib()
+        self.meshes = list(meshes)
+        self.h = self.L.bhg_series_create(int(device), len(self.meshes))
+        if not self.h:
+            raise HipError(f"no HIP device {device}")
+        for k, t in enumerate(self.meshes):
+            self._check(self.L.bhg_series_set_mesh(self.h, k, len(t.v1), _col(t.v1), _col(t.v2), _col(t.v3),
+                                                   float(t.truearea), float(t.truevol)), "set_mesh")
+
+    def close(self):
+        if self.h:
+            self.L.bhg_series_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise HipError(f"{what}: {self.L.bhg_series_last_error(self.h).decode(errors='replace')}")
+
+    def autotune(self, iarea=1, cg=1.0, ivisco=1):
+        self._check(self.L.bhg_series_autotune(self.h, int(iarea), float(cg), int(ivisco)), "autotune")
+        return self.tuning()
+
+    def set_tuning(self, k, strip=64, gemm_bk=32, ir_iters=4):
+        self._check(self.L.bhg_series_set_tuning(self.h, int(k), int(strip), int(gemm_bk), int(ir_iters)), "set_tuning")
+
+    def tuning(self):
+        out = []
+        for k in range(len(self.meshes)):
+            t = np.zeros(4, dtype=np.int32)
+            self.L.bhg_series_get_tuning(self.h, k, t)
+            out.append(dict(strip=int(t[0]), gemm_bk=int(t[1]), ir_iters=int(t[2]), parallel=int(t[3])))
+        return out
+
+    def solve(self, iarea=1, cg=1.0, ivisco=1, icov=0, wm=None):
+        """Results per mesh (dicts as Solver.solve_mesh) and the device time of the launch (ms)."""
+        res = (BhResult * len(self.meshes))()
+        ms = C.c_double()
+        self._check(self.L.bhg_series_solve(self.h, int(iarea), float(cg), int(ivisco), int(icov), res, C.byref(ms)),
+                    "solve")
+        out = []
+        for r in res:
+            d = r.as_dict()
+            if wm is not None and ivisco:
+                d["eta"] = d["chiprime"] * 0.602214 / wm
+            out.append(d)
+        return out, ms.value
+
+    def stats(self, k):
+        s = BhStats()
+        self.L.bhg_series_get_stats(self.h, int(k), C.byref(s))
+        return s.as_dict()
+
+
+def extrapolate_1n(n, y):
+    """(intercept, slope) of the least-squares line y = intercept + slope / N (bhg_extrapolate_1n)."""
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    a, b = C.c_double(), C.c_double()
+    if lib().bhg_extrapolate_1n(len(n), n, y, C.byref(a), C.byref(b)) != 0:
+        raise HipError("extrapolate_1n: fewer than two meshes or equal N")
+    return a.value, b.value
 
 
 def hm_plan(v1, v2, v3, x, leaf=32, eta=2.0):

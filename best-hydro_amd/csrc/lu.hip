@@ -53,8 +53,8 @@ __device__ inline T tabs( T v ) {
 // 16 t + i at slot 4 i + t, so the four fragments a lane needs for one k come as one 16-byte
 // (FP32) or two 16-byte (FP64) reads.  With beta != 0 the C tile is loaded into registers before
 // the main loop.  C may alias A (when N == 128: each workgroup reads its whole row strip before
-// it writes) or B (when M == 128).  check != nullptr: set *check when any |C| > 1 (the pivot test
-// of the panel below the diagonal block).
+// it writes) or B (when M == 128).  check != nullptr: set *check when any |C| > thr or is not
+// finite (the pivot test of the panel below the diagonal block: thr = 1 is partial pivoting's).
 // Batched over blockIdx.z: batch z = zdiv * q + t offsets A by q sa.hi + t sa.lo, and likewise B, C.
 struct BatchStride {
    long hi = 0, lo = 0;
@@ -62,7 +62,7 @@ struct BatchStride {
 
 template <typename T, int BK>
 __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
-                                                T beta, T* C, long ldc, int* check, int zdiv, BatchStride sa,
+                                                T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
                                                 BatchStride sb, BatchStride sc ) {
    constexpr int BM = 128, BN = 128, PAD = 8;
    constexpr int LDW = BM + PAD;  // LDS row length (elements)
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
             T v = alpha * acc[ t ][ u ][ r ];
             if ( beta != T( 0 ) ) v = v + beta * ( kPre ? cpre[ t ][ u ][ r ] : c[ crow( t, r ) ] );
             c[ crow( t, r ) ] = v;
-            big |= tabs( v ) > T( 1 );
+            big |= !( tabs( v ) <= thr );
          }
       }
    if ( check && big ) atomicOr( check, 1 );
@@ -245,9 +245,10 @@ __device__ __forceinline__ void tile_load( typename Mfma<T>::v4 ( &acc )[ 2 ][ 2
             acc[ bi ][ bj ][ r ] = C[ ( 16 * bj + ( lane & 15 ) ) * kLd + 16 * bi + acc_row<T>( lane, r ) ];
 }
 
-// Stores the tile; returns true if any |entry| > 1 (or is NaN) when check is set.
+// Stores the tile; returns true if any |entry| > thr (or is not finite) when check is set.
 template <typename T>
-__device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[ 2 ][ 2 ], T* C, int lane, bool check ) {
+__device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[ 2 ][ 2 ], T* C, int lane, bool check,
+                                            T thr = T( 1 ) ) {
    bool big = false;
 #pragma unroll
    for ( int bi = 0; bi < 2; ++bi )
@@ -256,7 +257,7 @@ __device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[
 #pragma unroll
          for ( int r = 0; r < 4; ++r ) {
             const T v = acc[ bi ][ bj ][ r ];
-            if ( check ) big |= !( tabs( v ) <= T( 1 ) );
+            if ( check ) big |= !( tabs( v ) <= thr );
             C[ ( 16 * bj + ( lane & 15 ) ) * kLd + 16 * bi + acc_row<T>( lane, r ) ] = v;
          }
    return big;
@@ -265,16 +266,17 @@ __device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[
 // One wave factors the 32 x 32 tile held as rows in registers (lane i and i + 32: row i, a[c] =
 // entry (i, c)) without exchanges: multipliers a_ij * (1 / a_jj), as DGETF2 scales; pivot-row
 // entries come from the pivot lane by readlane, so the 32 steps need no barrier.  Returns true
-// if some |a_ij| > |a_jj| at the step that eliminates column j (partial pivoting would exchange).
+// if some |a_ij| > thr |a_jj| at the step that eliminates column j (thr = 1: partial pivoting
+// would exchange rows).
 template <typename T>
-__device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i ) {
+__device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i, T thr ) {
    bool big = false;
 #pragma unroll
    for ( int j = 0; j < 32; ++j ) {
       const T piv = bcast( a[ j ], j );
       const T rp = T( 1 ) / piv;
       const bool below = i > j;
-      big |= below && !( tabs( a[ j ] ) <= tabs( piv ) );
+      big |= below && !( tabs( a[ j ] ) <= thr * tabs( piv ) );
       const T l = below ? a[ j ] * rp : T( 0 );
 #pragma unroll
       for ( int c = j + 1; c < 32; ++c ) a[ c ] = a[ c ] - l * bcast( a[ c ], j );
@@ -336,7 +338,7 @@ __device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i ) {
 // recursive doubling, inv([A 0; C B]) = [A^-1 0; -B^-1 C A^-1  B^-1] (and the upper analogue),
 // in place in LDS: 32 -> 64 (four independent tile tasks), then 64 -> 128.
 template <typename T>
-__global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T* __restrict__ inv, int* check ) {
+__global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T* __restrict__ inv, int* check, T thr ) {
    typedef typename Mfma<T>::v4 v4;
    __shared__ T S[ kNB * kLd ];
    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, i = lane & 31;
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
          if ( wave == 0 ) {
 #pragma unroll
             for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
-            big |= leaf_lu( a, i );
+            big |= leaf_lu( a, i, thr );
             if ( lane < 32 ) {
                T* Ap = A + (long)( 32 * p ) * lda + 32 * p;
 #pragma unroll
@@ -403,7 +405,7 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
             if ( t < nr ) {  // L panel: A U_pp^-1 (the multipliers: checked against 1)
                T* C = tile( p + 1 + t, p );
                tile_mma<T, kOpFull, kOpUpper>( acc, C, Spp, T( 1 ), lane );
-               big |= tile_store<T>( acc, C, lane, true );
+               big |= tile_store<T>( acc, C, lane, true, thr );
             } else {  // U row: L_pp^-1 A
                T* C = tile( p, p + 1 + t - nr );
                tile_mma<T, kOpLowerUnit, kOpFull>( acc, Spp, C, T( 1 ), lane );
@@ -830,6 +832,15 @@ __global__ void k_f2d( long n, const float* __restrict__ a, double* __restrict__
 
 // ------------------------------------------------------------------ host side
 
+// K slice of the FP32 GEMM (16, 32 or 64), chosen per mesh by the series autotuner; every launch
+// made on this host thread uses it.
+thread_local int g_gemm_bk_f32 = 32;
+
+// The largest multiplier the exchange-free factorization accepts before the caller reruns it with
+// row exchanges: 1 is partial pivoting's own test (the factors are then DGETRF's); the FP32
+// factorization that iterative refinement corrects accepts more.  Set per call on this thread.
+thread_local double g_growth = 1.0;
+
 template <typename T>
 void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta,
                           T* C, long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb,
@@ -839,10 +850,16 @@ void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T*
    ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32, 2.0 * M * N * K * batch, st );
    if constexpr ( sizeof( T ) == 8 )
       hipLaunchKernelGGL( ( k_gemm<double, 16> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
-                          zdiv, sa, sb, sc );
+                          T( g_growth ), zdiv, sa, sb, sc );
+   else if ( g_gemm_bk_f32 == 16 || K % 32 != 0 )
+      hipLaunchKernelGGL( ( k_gemm<float, 16> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
+                          T( g_growth ), zdiv, sa, sb, sc );
+   else if ( g_gemm_bk_f32 == 64 && K % 64 == 0 )
+      hipLaunchKernelGGL( ( k_gemm<float, 64> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
+                          T( g_growth ), zdiv, sa, sb, sc );
    else
       hipLaunchKernelGGL( ( k_gemm<float, 32> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
-                          zdiv, sa, sb, sc );
+                          T( g_growth ), zdiv, sa, sb, sc );
 }
 
 template <typename T>
@@ -874,7 +891,7 @@ void lu_nopiv( hipStream_t st, hipStream_t st2, hipEvent_t ev_u, hipEvent_t ev_g
    auto inv_u = [&]( int k ) { return inv + (long)k * 2 * nb * nb + nb * nb; };
    auto diag = [&]( int k ) {
       ProfScope prof( kKDiag, 4.0 / 3.0 * nb * nb * nb, st );  // LU 2/3 nb^3 + two triangular inverses nb^3 / 3 each
-      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, blk( k, k ), lda, inv_l( k ), check );
+      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, blk( k, k ), lda, inv_l( k ), check, T( g_growth ) );
    };
    auto lpanel = [&]( int k ) {  // the panel below the diagonal block: A U^-1 (checks |l| <= 1)
       launch_gemm<T>( st, npad - ( k + 1 ) * nb, nb, nb, T( 1 ), blk( k + 1, k ), lda, inv_u( k ), nb, T( 0 ),
@@ -939,7 +956,7 @@ void lu_pivoted( hipStream_t st, int npad, T* A, long lda, T* inv, int* ipiv, in
       }
       T* Akk = A + (long)kk * lda + kk;
       T* invk = inv + (long)k * 2 * nb * nb;
-      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 1, Akk, lda, invk, nullptr );
+      hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 1, Akk, lda, invk, nullptr, T( 1 ) );
       const int rest = npad - kk - nb;
       if ( rest <= 0 ) break;
       T* Abelow = Akk + nb;

@@ -53,6 +53,13 @@ void residual( hipStream_t, int, int, int, const double*, long, const double*, l
 size_t residual_part_elems( int );
 void ir_update( hipStream_t, int, int, double*, long, const float*, long, double* );
 void convert_d2f( hipStream_t, long, const double*, float* );
+extern thread_local int g_gemm_bk_f32;
+extern thread_local double g_growth;
+
+// Largest multiplier the exchange-free FP32 factorization accepts: iterative refinement against
+// the FP64 matrix removes the effect of the larger growth (the FP64 factorization keeps partial
+// pivoting's own test, 1, so its factors are DGETRF's).
+constexpr double kFp32Growth = 256.0;
 void convert_f2d( hipStream_t, long, const float*, double* );
 
 
@@ -139,6 +146,20 @@ namespace {
 
 int round_up( int v, int m ) { return ( v + m - 1 ) / m * m; }
 
+// Fills by kernel rather than hipMemsetAsync, so the solve can be captured in a HIP graph.
+__global__ void k_fill_d( long n, double* __restrict__ p, double v ) {
+   for ( long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x ) p[ i ] = v;
+}
+__global__ void k_fill_i( long n, int* __restrict__ p, int v ) {
+   for ( long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x ) p[ i ] = v;
+}
+void fill( hipStream_t st, double* p, long n, double v ) {
+   hipLaunchKernelGGL( k_fill_d, dim3( (unsigned)std::min<long>( 2048, ( n + 255 ) / 256 ) ), dim3( 256 ), 0, st, n, p, v );
+}
+void fill( hipStream_t st, int* p, long n, int v ) {
+   hipLaunchKernelGGL( k_fill_i, dim3( (unsigned)std::min<long>( 2048, ( n + 255 ) / 256 ) ), dim3( 256 ), 0, st, n, p, v );
+}
+
 void set_device( bhg_solver* s ) { BH_HIP( hipSetDevice( s->device ) ); }
 
 // Sizes the per-mesh buffers for n triangles.
@@ -199,7 +220,7 @@ void factor( bhg_solver* s, int precision ) {
    const size_t np = s->npad;
    const int nblk = s->npad / kNB;
    s->ipiv.ensure( np );
-   BH_HIP( hipMemsetAsync( s->flags.p, 0, 4 * sizeof( int ), s->st ) );
+   fill( s->st, s->flags.p, 4, 0 );
    s->pivoted = false;
    if ( precision == 64 ) {
       s->inv64.ensure( (size_t)nblk * 2 * kNB * kNB );
@@ -207,7 +228,7 @@ void factor( bhg_solver* s, int precision ) {
       if ( read_flag( s ) ) {
          s->say( "LU: a multiplier exceeded 1; refactoring with row exchanges" );
          assemble( s, s->deflated );
-         BH_HIP( hipMemsetAsync( s->flags.p, 0, 4 * sizeof( int ), s->st ) );
+         fill( s->st, s->flags.p, 4, 0 );
          lu_pivoted<double>( s->st, s->npad, s->a64.p, (long)np, s->inv64.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
          s->pivoted = true;
       }
@@ -219,11 +240,13 @@ void factor( bhg_solver* s, int precision ) {
       s->a32.ensure( np * np );
       s->inv32.ensure( (size_t)nblk * 2 * kNB * kNB );
       convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
+      g_growth = kFp32Growth;
       lu_nopiv<float>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
+      g_growth = 1.0;
       if ( read_flag( s ) ) {
-         s->say( "LU (FP32): a multiplier exceeded 1; refactoring with row exchanges" );
+         s->say( "LU (FP32): a multiplier exceeded 256; refactoring with row exchanges" );
          convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
-         BH_HIP( hipMemsetAsync( s->flags.p, 0, 4 * sizeof( int ), s->st ) );
+         fill( s->st, s->flags.p, 4, 0 );
          lu_pivoted<float>( s->st, s->npad, s->a32.p, (long)np, s->inv32.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
          s->pivoted = true;
       }
@@ -288,6 +311,69 @@ void size_vectors( bhg_solver* s ) {
    }
 }
 
+// One refinement step of the FP32 path: x0 = LU32^-1 b (it == 0), else r = b - A x in FP64 and
+// x += LU32^-1 r; norms[k] = max |dx|, norms[16 + k] = max |x| per column.  No host interaction.
+void ir_step( bhg_solver* s, int nrhs, int it ) {
+   const long np = s->npad;
+   fill( s->st, s->norms.p, 32, 0.0 );
+   if ( it == 0 ) {
+      fill( s->st, s->x.p, np * nrhs, 0.0 );
+      to_interleaved<double, float>( s->st, s->npad, nrhs, s->b.p, np, s->d32.p );
+   } else {
+      residual( s->st, s->npad, s->n3, nrhs, s->a64.p, np, s->x.p, np, s->b.p, np, s->r.p, np, s->part.p, s->norms.p );
+      to_interleaved<double, float>( s->st, s->npad, nrhs, s->r.p, np, s->d32.p );
+      fill( s->st, s->norms.p, 32, 0.0 );
+   }
+   lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, s->pivoted, s->d32.p, s->y32.p,
+                    s->x32.p, s->p32.p );
+   ir_update( s->st, s->n3, nrhs, s->x.p, np, s->x32.p, np, s->norms.p );
+}
+
+// max over the columns of |dx| / |x| from the last step's norms (synchronizes).
+double ir_correction( bhg_solver* s, int nrhs ) {
+   double hn[ 32 ];
+   BH_HIP( hipMemcpyAsync( hn, s->norms.p, sizeof hn, hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipStreamSynchronize( s->st ) );
+   double worst = 0.0;
+   for ( int k = 0; k < nrhs; ++k ) {
+      const double rel = hn[ 16 + k ] > 0 ? hn[ k ] / hn[ 16 + k ] : 0.0;
+      if ( !std::isfinite( rel ) ) throw std::runtime_error( "iterative refinement produced a non-finite correction" );
+      worst = std::max( worst, rel );
+   }
+   return worst;
+}
+
+// FP64 iterative refinement with the FP32 factors from step it0 on, until the correction is below
+// ir_tol relative to x in every column (or stagnates at the FP64 floor).
+// Returns false if the correction stagnated above kIrFloor (the FP32 factors too poor for the
+// refinement to converge).
+constexpr double kIrFloor = 1e-10;
+bool ir_loop( bhg_solver* s, int nrhs, int it0 ) {
+   double prev = 1e300, worst = 1e300;
+   for ( int it = it0; it < s->ir_maxit; ++it ) {
+      ir_step( s, nrhs, it );
+      worst = ir_correction( s, nrhs );
+      s->stats.ir_iterations = it + 1;
+      s->stats.ir_correction = worst;
+      if ( it > 0 && worst < s->ir_tol ) return true;
+      if ( it > 2 && worst > 0.5 * prev ) break;  // stagnation
+      prev = worst;
+   }
+   return worst < kIrFloor;
+}
+
+// The FP32 factors of the resident (deflated) matrix with row exchanges, after the exchange-free
+// ones did not let the refinement converge.
+void refactor_pivoted32( bhg_solver* s ) {
+   const size_t np = s->npad;
+   s->say( "LU (FP32): refinement did not converge with the exchange-free factors; refactoring with row exchanges" );
+   convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
+   fill( s->st, s->flags.p, 4, 0 );
+   lu_pivoted<float>( s->st, s->npad, s->a32.p, (long)np, s->inv32.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
+   s->pivoted = true;
+   build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, (long)np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
+}
+
 // Solves for the nrhs columns of s->b (npad x nrhs, rows >= n3 zero) into s->x.
 void solve_device( bhg_solver* s, int nrhs ) {
    const long np = s->npad;
@@ -307,37 +393,10 @@ void solve_device( bhg_solver* s, int nrhs ) {
       return;
    }
    if ( s->factored != 32 ) throw std::runtime_error( "solve before the factorization" );
-   // FP64 iterative refinement with the FP32 factors: x0 = LU32^-1 b, then r = b - A x,
-   // x += LU32^-1 r until the correction is below ir_tol relative to x in every column.
-   double hn[ 32 ];
-   double prev = 1e300;
-   for ( int it = 0; it < s->ir_maxit; ++it ) {
-      BH_HIP( hipMemsetAsync( s->norms.p, 0, 32 * sizeof( double ), s->st ) );
-      if ( it == 0 ) {
-         BH_HIP( hipMemsetAsync( s->x.p, 0, sizeof( double ) * np * nrhs, s->st ) );
-         to_interleaved<double, float>( s->st, s->npad, nrhs, s->b.p, np, s->d32.p );
-      } else {
-         residual( s->st, s->npad, s->n3, nrhs, s->a64.p, np, s->x.p, np, s->b.p, np, s->r.p, np, s->part.p, s->norms.p );
-         to_interleaved<double, float>( s->st, s->npad, nrhs, s->r.p, np, s->d32.p );
-         BH_HIP( hipMemsetAsync( s->norms.p, 0, 32 * sizeof( double ), s->st ) );
-      }
-      lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, s->pivoted, s->d32.p,
-                       s->y32.p, s->x32.p, s->p32.p );
-      ir_update( s->st, s->n3, nrhs, s->x.p, np, s->x32.p, np, s->norms.p );
-      BH_HIP( hipMemcpyAsync( hn, s->norms.p, sizeof hn, hipMemcpyDeviceToHost, s->st ) );
-      BH_HIP( hipStreamSynchronize( s->st ) );
-      double worst = 0.0;
-      for ( int k = 0; k < nrhs; ++k ) {
-         const double rel = hn[ 16 + k ] > 0 ? hn[ k ] / hn[ 16 + k ] : 0.0;
-         if ( !std::isfinite( rel ) ) throw std::runtime_error( "iterative refinement produced a non-finite correction" );
-         worst = std::max( worst, rel );
-      }
-      s->stats.ir_iterations = it + 1;
-      s->stats.ir_correction = worst;
-      if ( it > 0 && worst < s->ir_tol ) break;
-      if ( it > 2 && worst > 0.5 * prev ) break;  // stagnation at the FP64 floor
-      prev = worst;
-   }
+   if ( ir_loop( s, nrhs, 0 ) ) return;
+   if ( s->pivoted ) throw std::runtime_error( "iterative refinement did not converge" );
+   refactor_pivoted32( s );
+   if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
 }
 
 // Host-ordered sum of the per-workgroup partials.
@@ -350,6 +409,136 @@ void collect_sums( bhg_solver* s, double* out ) {
       for ( int g = 0; g < kSumGroups; ++g ) t = t + s->host_sums[ (size_t)g * kSumSlots + k ];
       out[ k ] = t;
    }
+}
+
+// The DIFCO/VISCO reductions of the solutions in s->x and BEST's algebra on them (best/difco.f,
+// best/visco.f, best/output.f) into out.  Synchronizes.
+void compute_results( bhg_solver* s, int iarea, double cg, int ivisco, int icov, bhg_result* out ) {
+   const int n = s->n;
+   const long np = s->npad;
+   const double zero[ 3 ] = { 0.0, 0.0, 0.0 };
+   launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, ivisco ? 5 : 0, zero, icov ? 1 : 0, s->sums.p );
+   double sums[ kSumSlots ];
+   collect_sums( s, sums );
+   Transport tr {};
+   difco_algebra( sums, cg, tr );
+   if ( ivisco ) {
+      if ( icov ) {
+         visco_centre( sums, tr.dvisc );
+         launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, 5, tr.dvisc, 0, s->sums.p );
+         double sums2[ kSumSlots ];
+         collect_sums( s, sums2 );
+         tr.chiprime = visco_chiprime( sums2, cg );
+      } else {
+         tr.chiprime = visco_chiprime( sums, cg );
+      }
+   }
+   output_values( tr );
+   double scal[ 8 ];
+   BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipStreamSynchronize( s->st ) );
+   if ( !out ) return;
+   std::memcpy( out->dtt, tr.dtt, sizeof tr.dtt );
+   std::memcpy( out->dtr, tr.dtr, sizeof tr.dtr );
+   std::memcpy( out->drr, tr.drr, sizeof tr.drr );
+   std::memcpy( out->urc, tr.urc, sizeof tr.urc );
+   std::memcpy( out->udr, tr.udr, sizeof tr.udr );
+   std::memcpy( out->ktt, tr.ktt, sizeof tr.ktt );
+   std::memcpy( out->krt, tr.krt, sizeof tr.krt );
+   std::memcpy( out->krr, tr.krr, sizeof tr.krr );
+   std::memcpy( out->drr_eig, tr.drr_eig, sizeof tr.drr_eig );
+   std::memcpy( out->dtt_eig, tr.dtt_eig, sizeof tr.dtt_eig );
+   out->dr = tr.dr;
+   out->dtrans = tr.dtrans;
+   out->chiprime = tr.chiprime;
+   out->chi = ( iarea == 1 && s->truevol != 0.0 ) ? tr.chiprime / s->truevol : 0.0;
+   std::memcpy( out->dvisc, tr.dvisc, sizeof tr.dvisc );
+   out->afac = scal[ 0 ];
+   out->totarea = scal[ 1 ];
+   std::memcpy( out->centroid, scal + 2, 3 * sizeof( double ) );
+   out->diagm = scal[ 5 ];
+   out->ncomp = s->ncomp;
+   out->nopen = s->nopen;
+   out->ndef = s->ndef;
+}
+
+// Tile choices of one mesh of a series (picked by the autotuner).
+struct MeshTune {
+   int strip = kPairStrip;  // source triangles per workgroup of the pair kernel
+   int gemm_bk = 32;        // K slice of the FP32 GEMM
+   int ir_iters = 4;        // refinement steps in the captured schedule
+};
+
+// Every buffer of the FP32 path for the resident mesh, so that nothing is allocated once a
+// schedule is captured.
+void prepare_buffers( bhg_solver* s ) {
+   const size_t np = s->npad;
+   const int nblk = s->npad / kNB;
+   s->hm.reset();
+   s->a64.ensure( np * np );
+   s->a32.ensure( np * np );
+   s->inv32.ensure( (size_t)nblk * 2 * kNB * kNB );
+   s->xl32.ensure( np * kSuper );
+   s->xu32.ensure( np * kSuper );
+   s->tmp32.ensure( np * kSuper / 2 );
+   s->ipiv.ensure( np );
+   s->gdef.ensure( std::max( s->ndef, 1 ) );
+   size_vectors( s );
+}
+
+// The whole FP32-LU + FP64-refinement solve of the resident mesh, enqueued on s->st (and s->st2)
+// with no host interaction: nothing here reads a device value, allocates or sets memory, so the
+// sequence can be captured in a HIP graph.  The pivot check and the refinement's convergence are
+// read afterwards (finish_enqueued).
+void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshTune& t ) {
+   const int n = s->n;
+   const long np = s->npad;
+   g_gemm_bk_f32 = t.gemm_bk;
+   s->cg = cg;
+   BH_HIP( hipMemcpyAsync( s->v.p, s->raw_v.p, 9 * (size_t)n * sizeof( double ), hipMemcpyDeviceToDevice, s->st ) );
+   launch_prep( s->st, n, V1( s ), V2( s ), V3( s ), iarea, s->truearea, s->area.p, s->scal.p );
+   self_terms( s, true, false );
+   launch_pairs( s->st, n, t.strip, s->cg, s->src.p, s->xc.p, s->selfb.p, s->xmu.p, s->comp.p, s->gdef.p, 1, s->a64.p, np );
+   launch_pad_identity( s->st, s->n3, s->npad, s->a64.p, np );
+   s->deflated = true;
+   fill( s->st, s->flags.p, 4, 0 );
+   convert_d2f( s->st, np * np, s->a64.p, s->a32.p );
+   g_growth = kFp32Growth;
+   lu_nopiv<float>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
+   g_growth = 1.0;
+   build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
+   launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
+   const int nrhs = ivisco ? 11 : 6;
+   for ( int it = 0; it < t.ir_iters; ++it ) ir_step( s, nrhs, it );
+   g_gemm_bk_f32 = 32;
+}
+
+// After an enqueued solve has run: if partial pivoting would have exchanged rows, the mesh is
+// solved again on the pivoted path; if the refinement has not converged it continues.  Then the
+// reductions and BEST's algebra.
+void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov, const MeshTune& t, bhg_result* out ) {
+   const int nrhs = ivisco ? 11 : 6;
+   s->factored = 32;
+   s->pivoted = false;
+   s->mesh_ready = true;
+   if ( read_flag( s ) ) {
+      s->say( "series: a multiplier exceeded 256; the mesh is solved again with row exchanges" );
+      assemble( s, true );
+      factor( s, 32 );
+      launch_rhs( s->st, s->n, s->xc.p, s->b.p, s->npad, ivisco ? 1 : 0 );
+      solve_device( s, nrhs );
+   } else {
+      const double c = ir_correction( s, nrhs );
+      s->stats.ir_iterations = t.ir_iters;
+      s->stats.ir_correction = c;
+      if ( !( c < s->ir_tol ) && !ir_loop( s, nrhs, t.ir_iters ) ) {
+         refactor_pivoted32( s );
+         if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
+      }
+   }
+   s->stats.pivoted = s->pivoted ? 1 : 0;
+   s->stats.npad = s->npad;
+   compute_results( s, iarea, cg, ivisco, icov, out );
 }
 
 }  // namespace
@@ -561,27 +750,8 @@ int bhg_solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, i
       bhg::launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
       bhg::solve_device( s, nrhs );
       T.mark( s->st, 4 );
-      const double zero[ 3 ] = { 0.0, 0.0, 0.0 };
-      bhg::launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, ivisco ? 5 : 0, zero, icov ? 1 : 0, s->sums.p );
-      double sums[ bhg::kSumSlots ];
-      bhg::collect_sums( s, sums );
-      bhg::Transport tr {};
-      bhg::difco_algebra( sums, cg, tr );
-      if ( ivisco ) {
-         if ( icov ) {
-            bhg::visco_centre( sums, tr.dvisc );
-            bhg::launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, 5, tr.dvisc, 0, s->sums.p );
-            double sums2[ bhg::kSumSlots ];
-            bhg::collect_sums( s, sums2 );
-            tr.chiprime = bhg::visco_chiprime( sums2, cg );
-         } else {
-            tr.chiprime = bhg::visco_chiprime( sums, cg );
-         }
-      }
-      bhg::output_values( tr );
+      bhg::compute_results( s, iarea, cg, ivisco, icov, out );
       T.mark( s->st, 5 );
-      double scal[ 8 ];
-      BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, s->st ) );
       BH_HIP( hipStreamSynchronize( s->st ) );
       s->stats.t_prep = T.ms( 0, 1 );
       s->stats.t_assemble = T.ms( 1, 2 );
@@ -592,30 +762,6 @@ int bhg_solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, i
       s->stats.pivoted = s->pivoted ? 1 : 0;
       s->stats.npad = s->npad;
       if ( s->prof.on ) s->prof.collect();
-      if ( out ) {
-         std::memcpy( out->dtt, tr.dtt, sizeof tr.dtt );
-         std::memcpy( out->dtr, tr.dtr, sizeof tr.dtr );
-         std::memcpy( out->drr, tr.drr, sizeof tr.drr );
-         std::memcpy( out->urc, tr.urc, sizeof tr.urc );
-         std::memcpy( out->udr, tr.udr, sizeof tr.udr );
-         std::memcpy( out->ktt, tr.ktt, sizeof tr.ktt );
-         std::memcpy( out->krt, tr.krt, sizeof tr.krt );
-         std::memcpy( out->krr, tr.krr, sizeof tr.krr );
-         std::memcpy( out->drr_eig, tr.drr_eig, sizeof tr.drr_eig );
-         std::memcpy( out->dtt_eig, tr.dtt_eig, sizeof tr.dtt_eig );
-         out->dr = tr.dr;
-         out->dtrans = tr.dtrans;
-         out->chiprime = tr.chiprime;
-         out->chi = ( iarea == 1 && s->truevol != 0.0 ) ? tr.chiprime / s->truevol : 0.0;
-         std::memcpy( out->dvisc, tr.dvisc, sizeof tr.dvisc );
-         out->afac = scal[ 0 ];
-         out->totarea = scal[ 1 ];
-         std::memcpy( out->centroid, scal + 2, 3 * sizeof( double ) );
-         out->diagm = scal[ 5 ];
-         out->ncomp = s->ncomp;
-         out->nopen = s->nopen;
-         out->ndef = s->ndef;
-      }
    } );
 }
 
@@ -878,6 +1024,271 @@ int bhg_besolv( bhg_solver* s, int nrhs, double* bh, long ldb ) {
 }
 
 // ---------------------------------------------------------------------- hmatrix/best_hmatrix.h
+
+// ------------------------------------------------------------------ series of meshes
+
+struct bhg_series {
+   int device = 0;
+   std::vector<bhg_solver*> m;
+   std::vector<bhg::MeshTune> tune;
+   hipStream_t so = nullptr;
+   std::vector<hipEvent_t> ev;  // fork and join per mesh
+   hipEvent_t t0 = nullptr, t1 = nullptr;
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t exec = nullptr;
+   int parallel = 1;            // meshes side by side in the schedule (1) or one after another (0)
+   int key_iarea = -1, key_ivisco = -1;
+   double key_cg = -1.0;
+   std::string last_error;
+   void drop_graph() {
+      if ( exec ) (void)hipGraphExecDestroy( exec );
+      if ( graph ) (void)hipGraphDestroy( graph );
+      exec = nullptr;
+      graph = nullptr;
+   }
+   ~bhg_series() {
+      drop_graph();
+      for ( auto e : ev ) (void)hipEventDestroy( e );
+      if ( t0 ) (void)hipEventDestroy( t0 );
+      if ( t1 ) (void)hipEventDestroy( t1 );
+      if ( so ) (void)hipStreamDestroy( so );
+      for ( auto p : m ) bhg_destroy( p );
+   }
+};
+
+}  // extern "C"
+
+namespace bhg {
+namespace {
+
+// Captures the solves of every mesh (enqueue_solve) as one HIP graph: forked from the series
+// stream to each mesh's streams and joined back, side by side or chained one after another.
+void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int parallel ) {
+   BH_HIP( hipSetDevice( S->device ) );
+   S->drop_graph();
+   for ( auto* m : S->m ) prepare_buffers( m );
+   BH_HIP( hipDeviceSynchronize() );
+   const int nm = (int)S->m.size();
+   BH_HIP( hipStreamBeginCapture( S->so, hipStreamCaptureModeRelaxed ) );
+   try {
+      hipStream_t prev = S->so;
+      for ( int k = 0; k < nm; ++k ) {
+         bhg_solver* m = S->m[ k ];
+         BH_HIP( hipEventRecord( S->ev[ 2 * k ], parallel ? S->so : prev ) );
+         BH_HIP( hipStreamWaitEvent( m->st, S->ev[ 2 * k ], 0 ) );
+         enqueue_solve( m, iarea, cg, ivisco, S->tune[ k ] );
+         BH_HIP( hipEventRecord( S->ev[ 2 * k + 1 ], m->st ) );
+         prev = m->st;
+      }
+      if ( parallel )
+         for ( int k = 0; k < nm; ++k ) BH_HIP( hipStreamWaitEvent( S->so, S->ev[ 2 * k + 1 ], 0 ) );
+      else
+         BH_HIP( hipStreamWaitEvent( S->so, S->ev[ 2 * nm - 1 ], 0 ) );
+   } catch ( ... ) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture( S->so, &g );
+      if ( g ) (void)hipGraphDestroy( g );
+      throw;
+   }
+   BH_HIP( hipStreamEndCapture( S->so, &S->graph ) );
+   BH_HIP( hipGraphInstantiate( &S->exec, S->graph, nullptr, nullptr, 0 ) );
+   S->parallel = parallel;
+   S->key_iarea = iarea;
+   S->key_cg = cg;
+   S->key_ivisco = ivisco;
+}
+
+// One launch of the captured schedule; the device time in ms.
+double series_replay( bhg_series* S ) {
+   BH_HIP( hipEventRecord( S->t0, S->so ) );
+   BH_HIP( hipGraphLaunch( S->exec, S->so ) );
+   BH_HIP( hipEventRecord( S->t1, S->so ) );
+   BH_HIP( hipEventSynchronize( S->t1 ) );
+   float ms = 0.f;
+   BH_HIP( hipEventElapsedTime( &ms, S->t0, S->t1 ) );
+   return ms;
+}
+
+template <class F>
+int series_guarded( bhg_series* S, const char* what, F&& f ) {
+   if ( !S ) return 1;
+   try {
+      f();
+      return 0;
+   } catch ( const std::exception& e ) {
+      S->last_error = std::string( what ) + ": " + e.what();
+      return 1;
+   }
+}
+
+}  // namespace
+}  // namespace bhg
+
+extern "C" {
+
+bhg_series* bhg_series_create( int device, int nmesh ) {
+   if ( nmesh < 1 ) return nullptr;
+   auto S = std::make_unique<bhg_series>();
+   S->device = device;
+   try {
+      BH_HIP( hipSetDevice( device ) );
+      BH_HIP( hipStreamCreateWithFlags( &S->so, hipStreamNonBlocking ) );
+      BH_HIP( hipEventCreate( &S->t0 ) );
+      BH_HIP( hipEventCreate( &S->t1 ) );
+      S->ev.resize( 2 * nmesh, nullptr );
+      for ( auto& e : S->ev ) BH_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+      for ( int k = 0; k < nmesh; ++k ) {
+         bhg_solver* m = bhg_create( device );
+         if ( !m ) throw std::runtime_error( "bhg_create" );
+         S->m.push_back( m );
+      }
+      S->tune.assign( nmesh, bhg::MeshTune {} );
+   } catch ( ... ) {
+      return nullptr;
+   }
+   return S.release();
+}
+
+void bhg_series_destroy( bhg_series* S ) { delete S; }
+
+const char* bhg_series_last_error( bhg_series* S ) { return S ? S->last_error.c_str() : "no series"; }
+
+int bhg_series_set_mesh( bhg_series* S, int k, int n, const double* v1, const double* v2, const double* v3, double truearea,
+                         double truevol ) {
+   return bhg::series_guarded( S, "bhg_series_set_mesh", [&] {
+      if ( k < 0 || k >= (int)S->m.size() ) throw std::runtime_error( "mesh index out of range" );
+      S->drop_graph();
+      if ( bhg_set_mesh( S->m[ k ], n, v1, v2, v3, truearea, truevol, nullptr ) != 0 )
+         throw std::runtime_error( S->m[ k ]->last_error );
+   } );
+}
+
+int bhg_series_set_tuning( bhg_series* S, int k, int strip, int gemm_bk, int ir_iters ) {
+   return bhg::series_guarded( S, "bhg_series_set_tuning", [&] {
+      if ( k < 0 || k >= (int)S->m.size() ) throw std::runtime_error( "mesh index out of range" );
+      if ( strip < 1 || ( gemm_bk != 16 && gemm_bk != 32 && gemm_bk != 64 ) || ir_iters < 1 )
+         throw std::runtime_error( "strip >= 1, gemm_bk 16, 32 or 64, ir_iters >= 1" );
+      S->tune[ k ] = bhg::MeshTune { strip, gemm_bk, ir_iters };
+      S->drop_graph();
+   } );
+}
+
+int bhg_series_get_tuning( bhg_series* S, int k, int* out ) {
+   if ( !S || k < 0 || k >= (int)S->m.size() ) return 1;
+   out[ 0 ] = S->tune[ k ].strip;
+   out[ 1 ] = S->tune[ k ].gemm_bk;
+   out[ 2 ] = S->tune[ k ].ir_iters;
+   out[ 3 ] = S->parallel;
+   return 0;
+}
+
+int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
+   return bhg::series_guarded( S, "bhg_series_autotune", [&] {
+      BH_HIP( hipSetDevice( S->device ) );
+      hipEvent_t e0, e1;
+      BH_HIP( hipEventCreate( &e0 ) );
+      BH_HIP( hipEventCreate( &e1 ) );
+      struct Ev {
+         hipEvent_t a, b;
+         ~Ev() {
+            (void)hipEventDestroy( a );
+            (void)hipEventDestroy( b );
+         }
+      } guard { e0, e1 };
+      for ( size_t k = 0; k < S->m.size(); ++k ) {
+         bhg_solver* m = S->m[ k ];
+         bhg::MeshTune t;
+         // an eager solve: allocations, and the refinement steps this mesh needs
+         if ( bhg_solve_mesh( m, iarea, cg, ivisco, 0, 32, nullptr ) != 0 ) throw std::runtime_error( m->last_error );
+         t.ir_iters = std::max( 2, m->stats.ir_iterations );
+         bhg::prepare_buffers( m );
+         const long np = m->npad;
+         auto time = [&]( auto&& f ) {
+            double best = 1e300;
+            for ( int rep = 0; rep < 2; ++rep ) {
+               BH_HIP( hipEventRecord( e0, m->st ) );
+               f();
+               BH_HIP( hipEventRecord( e1, m->st ) );
+               BH_HIP( hipEventSynchronize( e1 ) );
+               float ms = 0.f;
+               BH_HIP( hipEventElapsedTime( &ms, e0, e1 ) );
+               best = std::min( best, (double)ms );
+            }
+            return best;
+         };
+         double tb = 1e300;
+         for ( int strip : { 16, 32, 64, 128 } ) {
+            const double ms = time( [&] {
+               bhg::launch_pairs( m->st, m->n, strip, m->cg, m->src.p, m->xc.p, m->selfb.p, m->xmu.p, m->comp.p, m->gdef.p, 1,
+                                  m->a64.p, np );
+            } );
+            if ( ms < tb ) tb = ms, t.strip = strip;
+         }
+         tb = 1e300;
+         for ( int bk : { 16, 32, 64 } ) {
+            const double ms = time( [&] {
+               bhg::g_gemm_bk_f32 = bk;
+               bhg::fill( m->st, m->flags.p, 4, 0 );
+               bhg::convert_d2f( m->st, np * np, m->a64.p, m->a32.p );
+               bhg::g_growth = bhg::kFp32Growth;
+               bhg::lu_nopiv<float>( m->st, m->st2, m->ev_u, m->ev_g, m->npad, m->a32.p, np, m->inv32.p, m->flags.p );
+               bhg::g_growth = 1.0;
+               bhg::build_super_inverses<float>( m->st, m->npad, bhg::kSuper, m->a32.p, np, m->inv32.p, m->xl32.p, m->xu32.p,
+                                                 m->tmp32.p );
+               bhg::g_gemm_bk_f32 = 32;
+            } );
+            if ( ms < tb ) tb = ms, t.gemm_bk = bk;
+         }
+         S->tune[ k ] = t;
+      }
+      // side by side or one after another: the faster schedule
+      double best = 1e300;
+      int choice = 1;
+      for ( int par : { 1, 0 } ) {
+         bhg::series_capture( S, iarea, cg, ivisco, par );
+         bhg::series_replay( S );
+         const double ms = bhg::series_replay( S );
+         if ( ms < best ) best = ms, choice = par;
+      }
+      if ( choice != S->parallel ) bhg::series_capture( S, iarea, cg, ivisco, choice );
+   } );
+}
+
+int bhg_series_solve( bhg_series* S, int iarea, double cg, int ivisco, int icov, bhg_result* out, double* graph_ms ) {
+   return bhg::series_guarded( S, "bhg_series_solve", [&] {
+      for ( auto* m : S->m )
+         if ( !m->mesh_raw ) throw std::runtime_error( "a mesh of the series is not set" );
+      if ( !S->exec || S->key_iarea != iarea || S->key_cg != cg || S->key_ivisco != ivisco )
+         bhg::series_capture( S, iarea, cg, ivisco, S->parallel );
+      const double ms = bhg::series_replay( S );
+      if ( graph_ms ) *graph_ms = ms;
+      for ( size_t k = 0; k < S->m.size(); ++k )
+         bhg::finish_enqueued( S->m[ k ], iarea, cg, ivisco, icov, S->tune[ k ], out ? out + k : nullptr );
+   } );
+}
+
+int bhg_series_get_stats( bhg_series* S, int k, bhg_stats* out ) {
+   if ( !S || k < 0 || k >= (int)S->m.size() ) return 1;
+   return bhg_get_stats( S->m[ k ], out );
+}
+
+int bhg_extrapolate_1n( int nmesh, const int* n, const double* y, double* intercept, double* slope ) {
+   // least squares y = a + b / N
+   if ( nmesh < 2 ) return 1;
+   double sx = 0, sy = 0, sxx = 0, sxy = 0;
+   for ( int k = 0; k < nmesh; ++k ) {
+      const double x = 1.0 / n[ k ];
+      sx += x;
+      sy += y[ k ];
+      sxx += x * x;
+      sxy += x * y[ k ];
+   }
+   const double d = nmesh * sxx - sx * sx;
+   if ( d == 0.0 ) return 1;
+   *slope = ( nmesh * sxy - sx * sy ) / d;
+   *intercept = ( sy - *slope * sx ) / nmesh;
+   return 0;
+}
 
 static bhg_solver* g_solver = nullptr;
 static std::deque<std::string> g_log;

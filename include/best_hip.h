@@ -185,6 +185,42 @@ int bhg_hm_time_product( bhg_solver* s, int nrhs, int reps, double* ms, double* 
 /* y = (GR + deflation) x with the built H-matrix (hmatrix/src/hmatrix.cpp:441-471), BEST's order. */
 int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y, long ldy );
 
+/* ------------------------------------------------------------------ a series of meshes */
+
+/* The series of triangulations of one body at different N that rcoal makes so that BEST's results
+ * can be extrapolated linearly in 1/N (README.md:11-15): every mesh resident, the whole FP32-LU +
+ * FP64-refinement solve of all of them (assembly, factorization, refinement) captured as one HIP
+ * graph and launched at once.  No counterpart in the reference's call surface, which runs BEST
+ * once per mesh. */
+typedef struct bhg_series bhg_series;
+
+bhg_series* bhg_series_create( int device, int nmesh );
+void bhg_series_destroy( bhg_series* s );
+const char* bhg_series_last_error( bhg_series* s );
+
+/* Mesh k as BEST reads it (as bhg_set_mesh). */
+int bhg_series_set_mesh( bhg_series* s, int k, int n, const double* v1, const double* v2, const double* v3,
+                         double truearea, double truevol );
+
+/* The autotuner: for each mesh, the pair kernel's strip of source triangles per workgroup (16, 32,
+ * 64, 128) and the FP32 GEMM's K slice (16, 32, 64) that time fastest, the refinement steps an
+ * eager solve needed; then whether the meshes run side by side or one after another in the
+ * schedule, by timing both graphs.  Captures the schedule. */
+int bhg_series_autotune( bhg_series* s, int iarea, double cg, int ivisco );
+int bhg_series_set_tuning( bhg_series* s, int k, int strip, int gemm_bk, int ir_iters );
+/* out[0..3] = strip, gemm_bk, ir_iters, side by side (1) or chained (0). */
+int bhg_series_get_tuning( bhg_series* s, int k, int* out );
+
+/* One launch of the captured schedule (captured first if the options changed), then per mesh: the
+ * pivot check (a mesh that would need row exchanges is solved again on the pivoted path), more
+ * refinement steps if the correction is not yet below the tolerance, and the reductions; out[k]
+ * as bhg_solve_mesh's.  *graph_ms: the device time of the launch. */
+int bhg_series_solve( bhg_series* s, int iarea, double cg, int ivisco, int icov, bhg_result* out, double* graph_ms );
+int bhg_series_get_stats( bhg_series* s, int k, bhg_stats* out );
+
+/* Least-squares line y = intercept + slope / N through nmesh points: the 1/N extrapolation. */
+int bhg_extrapolate_1n( int nmesh, const int* n, const double* y, double* intercept, double* slope );
+
 #ifdef __cplusplus
 }
 #endif

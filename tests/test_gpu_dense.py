@@ -211,3 +211,20 @@ def test_large_mesh_precision_agreement(bh, solver):
     assert rel(r32["chiprime"], r64["chiprime"]) < TOL
     assert np.max(np.abs(s32 - s64)) < 1e-10 * np.max(np.abs(s64))
     assert solver.stats()["ir_iterations"] <= 8
+
+
+def test_row_exchanges_on_a_delaunay_mesh(bh, oracle, solver):
+    """A random-point Delaunay mesh whose GR needs 6 row exchanges under partial pivoting (LAPACK's
+    IPIV is not the identity): the FP64 LU takes the pivoted path and the FP32 LU accepts the
+    multipliers (up to 16) that refinement corrects; both within 1e-10 of the oracle."""
+    from best_hydro_amd import meshes as M
+    t = M.series_meshes(sizes=(2500,))[0]
+    ref = oracle.best_run(t.v1, t.v2, t.v3, t.truearea, t.truevol, iarea=1, cg=1.0, ivisco=1, icov=0)
+    solver.set_mesh(t)
+    r64 = solver.solve_mesh(precision=64)
+    assert solver.stats()["pivoted"] == 1
+    r32 = solver.solve_mesh(precision=32)
+    assert solver.stats()["pivoted"] == 0
+    for r in (r64, r32):
+        for key in ("dtrans", "dr", "chiprime"):
+            assert rel(r[key], ref[key]) < TOL, key
