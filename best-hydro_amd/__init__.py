@@ -165,6 +165,8 @@ def lib():
     L.bhg_hm_time_product.restype = i
     L.bhg_hm_apply.argtypes = [vp, i, _dp, lg, _dp, lg]
     L.bhg_hm_apply.restype = i
+    L.bhg_gradient.argtypes = [vp, i, d, i, C.POINTER(BhResult), _dp]
+    L.bhg_gradient.restype = i
     _LIB = L
     return L
 
@@ -232,6 +234,22 @@ class Solver:
         if wm is not None and ivisco:
             out["eta"] = out["chiprime"] * 0.602214 / wm
         return out
+
+    def gradient(self, iarea=1, cg=1.0, precision=64, wm=None):
+        """One solve of the resident mesh (strain flows on, icov 0) and the adjoint gradient.
+
+        Returns (values, grad): values as solve_mesh's; grad[o, i, v, c] = d(output o) /
+        d(coordinate c of vertex v of triangle i), outputs D_t (dtrans), D_r (dr), CHI V
+        (chiprime), and with wm also [eta] (0.602214 CHI V / wm) as a fourth row."""
+        res = BhResult()
+        g = np.zeros(27 * self.n)
+        self._check(self.L.bhg_gradient(self.h, int(iarea), float(cg), int(precision), C.byref(res), g), "gradient")
+        out = res.as_dict()
+        grad = g.reshape(3, 3, 3, self.n).transpose(0, 3, 1, 2).copy()
+        if wm is not None:
+            out["eta"] = out["chiprime"] * 0.602214 / wm
+            grad = np.concatenate([grad, grad[2:3] * (0.602214 / wm)])
+        return out, grad
 
     def solution(self, nrhs=11):
         out = np.zeros(3 * self.n * nrhs)

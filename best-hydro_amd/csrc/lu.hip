@@ -690,12 +690,14 @@ __global__ void k_from_interleaved( int n, int nrhs, const S* __restrict__ v, D*
    }
 }
 
-// Row interchanges of the factorization applied in order to a row-interleaved block.
+// Row interchanges of the factorization applied to a row-interleaved block: in order (P v), or
+// undone in reverse order (P^T v, the last step of the transposed solve).
 template <typename T>
-__global__ void k_apply_ipiv_il( int n, const int* __restrict__ ipiv, T* v ) {
+__global__ void k_apply_ipiv_il( int n, const int* __restrict__ ipiv, T* v, int reverse ) {
    const int r = threadIdx.x;
    if ( r >= 16 ) return;
-   for ( int k = 0; k < n; ++k ) {
+   for ( int j = 0; j < n; ++j ) {
+      const int k = reverse ? n - 1 - j : j;
       const int p = ipiv[ k ] - 1;
       if ( p != k ) {
          const T t = v[ (long)k * 16 + r ];
@@ -1020,11 +1022,14 @@ template void build_super_inverses<float>( hipStream_t, int, int, const float*, 
 // npad x 16 blocks: b is consumed (row-exchanged when pivoted, then overwritten by the forward
 // updates), y is work space, the solution goes to x.  Two launches per super-block and triangle:
 // the inverse times the block, then the update of the rest of the right-hand side.
+// piv: 0 no exchanges; 1 P b first (A = P^T L U); -1 the factors are those of A^T = U^T L^T P
+// transposed in place (the packed factors and the super-block inverses transposed, XL and XU
+// exchanged), so P^T is applied to the solution last.
 template <typename T>
 void lu_solve( hipStream_t st, int npad, int n3, int S, const T* LU, long lda, const T* XL, const T* XU, const int* ipiv,
-               bool pivoted, T* b, T* y, T* x, T* P ) {
+               int piv, T* b, T* y, T* x, T* P ) {
    const int nsb = npad / S;
-   if ( pivoted ) hipLaunchKernelGGL( k_apply_ipiv_il<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, ipiv, b );
+   if ( piv == 1 ) hipLaunchKernelGGL( k_apply_ipiv_il<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, ipiv, b, 0 );
    for ( int q = 0; q < nsb; ++q ) {
       const long q0 = (long)q * S;
       launch_skinny<T>( st, S, S, T( 1 ), XL + q * (long)S * S, S, b + q0 * 16, T( 0 ), y + q0 * 16, P );
@@ -1036,10 +1041,11 @@ void lu_solve( hipStream_t st, int npad, int n3, int S, const T* LU, long lda, c
       launch_skinny<T>( st, S, S, T( 1 ), XU + q * (long)S * S, S, y + q0 * 16, T( 0 ), x + q0 * 16, P );
       launch_skinny<T>( st, (int)q0, S, T( -1 ), LU + q0 * lda, lda, x + q0 * 16, T( 1 ), y, P );
    }
+   if ( piv == -1 ) hipLaunchKernelGGL( k_apply_ipiv_il<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, ipiv, x, 1 );
 }
 template void lu_solve<double>( hipStream_t, int, int, int, const double*, long, const double*, const double*, const int*,
-                                bool, double*, double*, double*, double* );
-template void lu_solve<float>( hipStream_t, int, int, int, const float*, long, const float*, const float*, const int*, bool,
+                                int, double*, double*, double*, double* );
+template void lu_solve<float>( hipStream_t, int, int, int, const float*, long, const float*, const float*, const int*, int,
                                float*, float*, float*, float* );
 
 template <typename S, typename D>

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/best_hip.h"
+#include "adjoint.h"
 #include "bh_common.h"
 #include "hmatrix.h"
 #include "transport.h"
@@ -43,7 +44,7 @@ void lu_pivoted( hipStream_t, int, T*, long, T*, int*, int*, double*, int* );
 template <typename T>
 void build_super_inverses( hipStream_t, int, int, const T*, long, const T*, T*, T*, T* );
 template <typename T>
-void lu_solve( hipStream_t, int, int, int, const T*, long, const T*, const T*, const int*, bool, T*, T*, T*, T* );
+void lu_solve( hipStream_t, int, int, int, const T*, long, const T*, const T*, const int*, int, T*, T*, T*, T* );
 template <typename S, typename D>
 void to_interleaved( hipStream_t, int, int, const S*, long, D* );
 template <typename S, typename D>
@@ -120,9 +121,13 @@ struct bhg_solver {
    bhg::HMatrixPtr hm;
    bool pivoted = false;      // the slow path ran (rows were exchanged)
    bool deflated = true;      // the factored matrix carries the deflation
+   bool transposed = false;   // the factors (and the FP64 matrix) are transposed in place: solves are with GR^T
    // vectors
    bhg::DevBuf<double> b, x, r, y64, x64il, p64, part, norms, sums;
    bhg::DevBuf<float> d32, y32, x32, p32;
+   // adjoint (bhg_gradient)
+   bhg::DevBuf<double> adj_F, adj_y, adj_lam, adj_ghat, adj_gsrc_part, adj_gcol_part, adj_gsrc, adj_gcol;
+   bhg::DevBuf<bhg::AdjTri> adj_tri;
    std::vector<double> host_sums;
    bhg::PhaseTimer timer;
    bhg::LaunchProfiler prof;
@@ -222,6 +227,7 @@ void factor( bhg_solver* s, int precision ) {
    s->ipiv.ensure( np );
    fill( s->st, s->flags.p, 4, 0 );
    s->pivoted = false;
+   s->transposed = false;
    if ( precision == 64 ) {
       s->inv64.ensure( (size_t)nblk * 2 * kNB * kNB );
       lu_nopiv<double>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a64.p, (long)np, s->inv64.p, s->flags.p );
@@ -285,6 +291,7 @@ void hm_factor( bhg_solver* s ) {
    for ( auto& l : lines ) s->log.push_back( std::move( l ) );
    s->factored = -1;
    s->pivoted = false;
+   s->transposed = false;
 }
 
 void size_vectors( bhg_solver* s ) {
@@ -311,6 +318,9 @@ void size_vectors( bhg_solver* s ) {
    }
 }
 
+// How lu_solve applies the row exchanges of the resident factors.
+int piv_mode( const bhg_solver* s ) { return s->pivoted ? ( s->transposed ? -1 : 1 ) : 0; }
+
 // One refinement step of the FP32 path: x0 = LU32^-1 b (it == 0), else r = b - A x in FP64 and
 // x += LU32^-1 r; norms[k] = max |dx|, norms[16 + k] = max |x| per column.  No host interaction.
 void ir_step( bhg_solver* s, int nrhs, int it ) {
@@ -324,7 +334,7 @@ void ir_step( bhg_solver* s, int nrhs, int it ) {
       to_interleaved<double, float>( s->st, s->npad, nrhs, s->r.p, np, s->d32.p );
       fill( s->st, s->norms.p, 32, 0.0 );
    }
-   lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, s->pivoted, s->d32.p, s->y32.p,
+   lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, piv_mode( s ), s->d32.p, s->y32.p,
                     s->x32.p, s->p32.p );
    ir_update( s->st, s->n3, nrhs, s->x.p, np, s->x32.p, np, s->norms.p );
 }
@@ -387,14 +397,14 @@ void solve_device( bhg_solver* s, int nrhs ) {
    }
    if ( s->factored == 64 ) {
       to_interleaved<double, double>( s->st, s->npad, nrhs, s->b.p, np, s->r.p );
-      lu_solve<double>( s->st, s->npad, s->n3, kSuper, s->a64.p, np, s->xl64.p, s->xu64.p, s->ipiv.p, s->pivoted, s->r.p,
+      lu_solve<double>( s->st, s->npad, s->n3, kSuper, s->a64.p, np, s->xl64.p, s->xu64.p, s->ipiv.p, piv_mode( s ), s->r.p,
                         s->y64.p, s->x64il.p, s->p64.p );
       from_interleaved<double, double>( s->st, s->n3, nrhs, s->x64il.p, s->x.p, np );
       return;
    }
    if ( s->factored != 32 ) throw std::runtime_error( "solve before the factorization" );
    if ( ir_loop( s, nrhs, 0 ) ) return;
-   if ( s->pivoted ) throw std::runtime_error( "iterative refinement did not converge" );
+   if ( s->pivoted || s->transposed ) throw std::runtime_error( "iterative refinement did not converge" );
    refactor_pivoted32( s );
    if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
 }
@@ -520,6 +530,7 @@ void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
    const int nrhs = ivisco ? 11 : 6;
    s->factored = 32;
    s->pivoted = false;
+   s->transposed = false;
    s->mesh_ready = true;
    if ( read_flag( s ) ) {
       s->say( "series: a multiplier exceeded 256; the mesh is solved again with row exchanges" );
@@ -539,6 +550,262 @@ void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
    s->stats.pivoted = s->pivoted ? 1 : 0;
    s->stats.npad = s->npad;
    compute_results( s, iarea, cg, ivisco, icov, out );
+}
+
+// One dense (or H-matrix) solve of the resident mesh: prep, self terms, assembly, factorization,
+// the 6 or 11 right-hand sides, reductions and algebra.
+void solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int precision, bhg_result* out ) {
+   if ( !s->mesh_raw ) throw std::runtime_error( "no mesh (bhg_set_mesh)" );
+   if ( precision != 32 && precision != 64 ) throw std::runtime_error( "precision must be 32 or 64" );
+   set_device( s );
+   const int n = s->n;
+   const long np = s->npad;
+   s->cg = cg;
+   struct ProfBind {
+      explicit ProfBind( LaunchProfiler* p ) { g_prof = p->on ? p : nullptr; }
+      ~ProfBind() { g_prof = nullptr; }
+   } bind( &s->prof );
+   s->prof.st = s->st;
+   auto& T = s->timer;
+   T.mark( s->st, 0 );
+   BH_HIP( hipMemcpyAsync( s->v.p, s->raw_v.p, 9 * (size_t)n * sizeof( double ), hipMemcpyDeviceToDevice, s->st ) );
+   launch_prep( s->st, n, V1( s ), V2( s ), V3( s ), iarea, s->truearea, s->area.p, s->scal.p );
+   self_terms( s, true, false );
+   s->mesh_ready = true;
+   T.mark( s->st, 1 );
+   if ( s->method == 1 ) {  // the H-matrix build is the factorization phase
+      T.mark( s->st, 2 );
+      hm_factor( s );
+   } else {
+      assemble( s, true );
+      T.mark( s->st, 2 );
+      factor( s, precision );
+   }
+   T.mark( s->st, 3 );
+   size_vectors( s );
+   const int nrhs = ivisco ? 11 : 6;
+   launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
+   solve_device( s, nrhs );
+   T.mark( s->st, 4 );
+   compute_results( s, iarea, cg, ivisco, icov, out );
+   T.mark( s->st, 5 );
+   BH_HIP( hipStreamSynchronize( s->st ) );
+   s->stats.t_prep = T.ms( 0, 1 );
+   s->stats.t_assemble = T.ms( 1, 2 );
+   s->stats.t_factor = T.ms( 2, 3 );
+   s->stats.t_solve = T.ms( 3, 4 );
+   s->stats.t_reduce = T.ms( 4, 5 );
+   s->stats.t_total = T.ms( 0, 5 );
+   s->stats.pivoted = s->pivoted ? 1 : 0;
+   s->stats.npad = s->npad;
+   if ( s->prof.on ) s->prof.collect();
+}
+
+
+// ------------------------------------------------------------------ adjoint
+
+// The differentiated outputs from the reductions (slots 0..44, no centre-of-viscosity move):
+// D_t and D_r as output_values prints them, CHI * V.
+void adjoint_outputs( const double* sums, double cg, double q[ kAdjOutputs ] ) {
+   Transport tr {};
+   difco_algebra( sums, cg, tr );
+   output_values( tr );
+   q[ 0 ] = tr.dtrans;
+   q[ 1 ] = tr.dr;
+   q[ 2 ] = visco_chiprime( sums, cg );
+}
+
+// ghat[o][q] = d(output o)/d(slot q) by fourth-order central differences of the host algebra (a
+// few hundred 3x3 evaluations).  The step is 1e-4 of the largest magnitude among the slots of
+// the same kind (force or torque of a translation or a rotation, the CHI moments), so that slots
+// that vanish by symmetry get a step of the right scale.
+void adjoint_weights( const double* sums, double cg, double* ghat ) {
+   auto kind = []( int q ) { return q < 36 ? 2 * ( q / 6 >= 3 ) + ( q % 6 >= 3 ) : 4; };
+   double scale[ 5 ] = {};
+   for ( int q = 0; q < kAdjSlots; ++q ) scale[ kind( q ) ] = std::max( scale[ kind( q ) ], std::fabs( sums[ q ] ) );
+   std::vector<double> t( sums, sums + kSumSlots );
+   for ( int q = 0; q < kAdjSlots; ++q ) {
+      const double h = 1e-4 * ( scale[ kind( q ) ] > 0 ? scale[ kind( q ) ] : 1.0 );
+      double f[ 4 ][ kAdjOutputs ];
+      const double off[ 4 ] = { -2 * h, -h, h, 2 * h };
+      for ( int e = 0; e < 4; ++e ) {
+         t[ q ] = sums[ q ] + off[ e ];
+         adjoint_outputs( t.data(), cg, f[ e ] );
+      }
+      t[ q ] = sums[ q ];
+      for ( int o = 0; o < kAdjOutputs; ++o )
+         ghat[ o * kAdjSlots + q ] = ( f[ 0 ][ o ] - 8.0 * f[ 1 ][ o ] + 8.0 * f[ 2 ][ o ] - f[ 3 ][ o ] ) / ( 12.0 * h );
+   }
+}
+
+template <typename T>
+void swap_bufs( DevBuf<T>& a, DevBuf<T>& b ) {
+   std::swap( a.p, b.p );
+   std::swap( a.n, b.n );
+}
+
+// Transposes the resident factorization in place: P GR = L U becomes the factors of
+// GR^T = U^T L^T P (the packed factors transposed, the super-block inverses of L and U
+// transposed and exchanged); on the FP32 path the FP64 matrix of the refinement residuals too.
+// Applied twice it restores the factors.
+void transpose_factors( bhg_solver* s ) {
+   const int np = s->npad, nsb = s->npad / kSuper;
+   const long SS = (long)kSuper * kSuper;
+   if ( s->factored == 64 ) {
+      launch_transpose<double>( s->st, np, s->a64.p, np, 1, 0 );
+      launch_transpose<double>( s->st, kSuper, s->xl64.p, kSuper, nsb, SS );
+      launch_transpose<double>( s->st, kSuper, s->xu64.p, kSuper, nsb, SS );
+      swap_bufs( s->xl64, s->xu64 );
+   } else if ( s->factored == 32 ) {
+      launch_transpose<double>( s->st, np, s->a64.p, np, 1, 0 );
+      launch_transpose<float>( s->st, np, s->a32.p, np, 1, 0 );
+      launch_transpose<float>( s->st, kSuper, s->xl32.p, kSuper, nsb, SS );
+      launch_transpose<float>( s->st, kSuper, s->xu32.p, kSuper, nsb, SS );
+      swap_bufs( s->xl32, s->xu32 );
+   } else {
+      throw std::runtime_error( "the gradient needs the dense factorization" );
+   }
+   s->transposed = !s->transposed;
+}
+
+// The gradient of D_t, D_r and CHI * V with respect to the raw vertex coordinates, after a dense
+// solve of the resident mesh with 11 right-hand sides (solutions in s->x):
+//   grad[o * 9 n + v * 3 n + c * n + i] = d(output o) / d(vertex v of triangle i, coordinate c).
+void gradient( bhg_solver* s, int iarea, double cg, double* grad ) {
+   const int n = s->n, n3 = s->n3;
+   const long np = s->npad;
+   constexpr int K = 11, O = kAdjOutputs;
+   hipStream_t st = s->st;
+   // solutions and the output weights of the reductions
+   s->adj_F.ensure( np * K );
+   BH_HIP( hipMemcpyAsync( s->adj_F.p, s->x.p, np * K * sizeof( double ), hipMemcpyDeviceToDevice, st ) );
+   const double zero[ 3 ] = { 0.0, 0.0, 0.0 };
+   launch_sums( st, n, s->xc.p, s->area.p, s->x.p, np, 5, zero, 0, s->sums.p );
+   double sums[ kSumSlots ];
+   collect_sums( s, sums );
+   double ghat[ O * kAdjSlots ];
+   adjoint_weights( sums, cg, ghat );
+   s->adj_ghat.ensure( O * kAdjSlots );
+   BH_HIP( hipMemcpyAsync( s->adj_ghat.p, ghat, sizeof ghat, hipMemcpyHostToDevice, st ) );
+   // adjoint right-hand sides and solves with the transposed factors
+   s->adj_y.ensure( np * K * O );
+   s->adj_lam.ensure( np * K * O );
+   fill( st, s->adj_y.p, np * K * O, 0.0 );
+   launch_adj_rhs( st, n, s->xc.p, s->area.p, s->adj_ghat.p, s->adj_y.p, np );
+   transpose_factors( s );
+   try {
+      for ( int o = 0; o < O; ++o ) {
+         BH_HIP( hipMemcpyAsync( s->b.p, s->adj_y.p + o * K * np, np * K * sizeof( double ), hipMemcpyDeviceToDevice, st ) );
+         solve_device( s, K );
+         BH_HIP( hipMemcpyAsync( s->adj_lam.p + o * K * np, s->x.p, np * K * sizeof( double ), hipMemcpyDeviceToDevice, st ) );
+      }
+   } catch ( ... ) {
+      transpose_factors( s );
+      throw;
+   }
+   transpose_factors( s );
+   BH_HIP( hipMemcpyAsync( s->x.p, s->adj_F.p, np * K * sizeof( double ), hipMemcpyDeviceToDevice, st ) );
+   // pair terms (source strips sized for at least 1024 workgroups) and per-triangle terms
+   const int nb = ( n + 255 ) / 256;
+   const int strip = std::max( 8, std::min( 128, (int)( ( (long)n * nb + 1023 ) / 1024 ) ) );
+   const int ns = ( n + strip - 1 ) / strip;
+   s->adj_gsrc_part.ensure( (size_t)nb * n * O * 10 );
+   s->adj_gcol_part.ensure( (size_t)ns * n * O * 3 );
+   s->adj_gsrc.ensure( (size_t)n * O * 10 );
+   s->adj_gcol.ensure( (size_t)n * O * 3 );
+   s->adj_tri.ensure( n );
+   launch_adj_pairs( st, n, strip, s->cg, s->src.p, s->xc.p, s->adj_lam.p, s->adj_F.p, np, s->adj_gsrc_part.p, s->adj_gcol_part.p,
+                     s->adj_gsrc.p, s->adj_gcol.p );
+   launch_adj_tri( st, n, V1( s ), V2( s ), V3( s ), s->scal.p, s->cg, s->adj_lam.p, s->adj_F.p, np, s->adj_ghat.p,
+                   s->adj_tri.p );
+   // to the host: the chain rule through the deflation, the area factor and the centroid shift
+   std::vector<double> gsrc( (size_t)n * O * 10 ), gcol( (size_t)n * O * 3 ), lam( (size_t)n3 * K * O ), F( (size_t)n3 * K );
+   std::vector<AdjTri> tri( n );
+   std::vector<int> comp( n );
+   const int ndef = std::max( s->ndef, 1 );
+   std::vector<double> gdef( ndef );
+   double scal[ 8 ];
+   BH_HIP( hipMemcpyAsync( gsrc.data(), s->adj_gsrc.p, gsrc.size() * sizeof( double ), hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpyAsync( gcol.data(), s->adj_gcol.p, gcol.size() * sizeof( double ), hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpyAsync( tri.data(), s->adj_tri.p, n * sizeof( AdjTri ), hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpy2DAsync( lam.data(), n3 * sizeof( double ), s->adj_lam.p, np * sizeof( double ), n3 * sizeof( double ), K * O,
+                             hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpy2DAsync( F.data(), n3 * sizeof( double ), s->adj_F.p, np * sizeof( double ), n3 * sizeof( double ), K,
+                             hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpyAsync( comp.data(), s->comp.p, n * sizeof( int ), hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpyAsync( gdef.data(), s->gdef.p, ndef * sizeof( double ), hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipStreamSynchronize( st ) );
+   const double afac = scal[ 0 ], tot = scal[ 1 ], diagm = scal[ 5 ];
+   auto Lm = [&]( int o, int k, int i, int c ) { return lam[ ( (size_t)o * K + k ) * n3 + 3 * (size_t)i + c ]; };
+   auto Fm = [&]( int k, int i, int c ) { return F[ (size_t)k * n3 + 3 * (size_t)i + c ]; };
+   auto surf = [&]( int i ) { return ndef > 1 ? comp[ i ] - 1 : 0; };
+   // deflation: per surface, X = sum |xmu|^2, alpha = lambda . xmu, beta = F . xmu, P = alpha . beta
+   std::vector<double> X( ndef, 0.0 ), alpha( (size_t)O * K * ndef, 0.0 ), beta( (size_t)K * ndef, 0.0 );
+   for ( int i = 0; i < n; ++i ) {
+      const int S = surf( i );
+      const double* xm = tri[ i ].xmu;
+      X[ S ] += xm[ 0 ] * xm[ 0 ] + xm[ 1 ] * xm[ 1 ] + xm[ 2 ] * xm[ 2 ];
+      for ( int k = 0; k < K; ++k ) {
+         beta[ k * ndef + S ] += Fm( k, i, 0 ) * xm[ 0 ] + Fm( k, i, 1 ) * xm[ 1 ] + Fm( k, i, 2 ) * xm[ 2 ];
+         for ( int o = 0; o < O; ++o )
+            alpha[ ( o * K + k ) * ndef + S ] += Lm( o, k, i, 0 ) * xm[ 0 ] + Lm( o, k, i, 1 ) * xm[ 1 ] + Lm( o, k, i, 2 ) * xm[ 2 ];
+      }
+   }
+   std::vector<double> P( (size_t)O * ndef, 0.0 );
+   double Pt[ O ] = {};
+   for ( int o = 0; o < O; ++o )
+      for ( int S = 0; S < ndef; ++S ) {
+         for ( int k = 0; k < K; ++k ) P[ o * ndef + S ] += alpha[ ( o * K + k ) * ndef + S ] * beta[ k * ndef + S ];
+         Pt[ o ] += gdef[ S ] * P[ o * ndef + S ];
+      }
+   const double ddiag = 1.0 / ( 3.0 * n * diagm );
+   double dafac[ O ] = {};
+   const size_t n9 = 9 * (size_t)n;
+   for ( int i = 0; i < n; ++i ) {
+      const AdjTri& R = tri[ i ];
+      const int S = surf( i );
+      const double g = gdef[ S ];
+      for ( int o = 0; o < O; ++o ) {
+         double w[ 3 ];
+         for ( int c = 0; c < 3; ++c ) {
+            double a = 0.0;
+            for ( int k = 0; k < K; ++k ) a += Lm( o, k, i, c ) * beta[ k * ndef + S ] + alpha[ ( o * K + k ) * ndef + S ] * Fm( k, i, c );
+            w[ c ] = a;
+         }
+         const double* gs = &gsrc[ ( (size_t)i * O + o ) * 10 ];
+         const double* gc = &gcol[ ( (size_t)i * O + o ) * 3 ];
+         for ( int d = 0; d < 10; ++d ) {
+            double xdx = 0.0, wdx = 0.0;
+            for ( int c = 0; c < 3; ++c ) xdx += R.xmu[ c ] * R.dxmu[ c ][ d ], wdx += w[ c ] * R.dxmu[ c ][ d ];
+            const double defl = Pt[ o ] * R.dtr[ d ] * ddiag - g * P[ o * ndef + S ] * 2.0 * xdx / X[ S ] + g * wdx;
+            if ( d == 9 ) {
+               dafac[ o ] += ( R.gsa[ o ] - gs[ 9 ] ) * R.A - R.gself[ o ][ 9 ] - defl;
+               continue;
+            }
+            const double dar = afac * R.dA[ d ];
+            double v = ( R.gsa[ o ] - gs[ 9 ] ) * dar - gs[ d ] - R.gself[ o ][ d ] - defl;
+            for ( int c = 0; c < 3; ++c ) v += ( R.gsx[ o ][ c ] + R.grhs[ o ][ c ] - gc[ c ] ) * R.dx[ c ][ d ];
+            grad[ o * n9 + ( d / 3 ) * 3 * (size_t)n + ( d % 3 ) * (size_t)n + i ] = v;
+         }
+      }
+   }
+   for ( int o = 0; o < O; ++o ) {
+      double* G = grad + o * n9;
+      // afac = truearea / (sum of the geometric areas) when the areas are corrected
+      if ( iarea == 1 )
+         for ( int i = 0; i < n; ++i )
+            for ( int d = 0; d < 9; ++d ) G[ ( d / 3 ) * 3 * (size_t)n + ( d % 3 ) * (size_t)n + i ] -= dafac[ o ] * afac / tot * tri[ i ].dA[ d ];
+      // the vertices are shifted by the mean of the triangle centroids first
+      for ( int c = 0; c < 3; ++c ) {
+         double m = 0.0;
+         for ( int v = 0; v < 3; ++v )
+            for ( int i = 0; i < n; ++i ) m += G[ v * 3 * (size_t)n + c * (size_t)n + i ];
+         m /= 3.0 * n;
+         for ( int v = 0; v < 3; ++v )
+            for ( int i = 0; i < n; ++i ) G[ v * 3 * (size_t)n + c * (size_t)n + i ] -= m;
+      }
+   }
 }
 
 }  // namespace
@@ -717,51 +984,15 @@ int bhg_set_mesh( bhg_solver* s, int n, const double* v1, const double* v2, cons
 }
 
 int bhg_solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int precision, bhg_result* out ) {
-   return guarded( s, "bhg_solve_mesh", [&] {
-      if ( !s->mesh_raw ) throw std::runtime_error( "no mesh (bhg_set_mesh)" );
-      if ( precision != 32 && precision != 64 ) throw std::runtime_error( "precision must be 32 or 64" );
-      bhg::set_device( s );
-      const int n = s->n;
-      const long np = s->npad;
-      s->cg = cg;
-      struct ProfBind {
-         explicit ProfBind( bhg::LaunchProfiler* p ) { bhg::g_prof = p->on ? p : nullptr; }
-         ~ProfBind() { bhg::g_prof = nullptr; }
-      } bind( &s->prof );
-      s->prof.st = s->st;
-      auto& T = s->timer;
-      T.mark( s->st, 0 );
-      BH_HIP( hipMemcpyAsync( s->v.p, s->raw_v.p, 9 * (size_t)n * sizeof( double ), hipMemcpyDeviceToDevice, s->st ) );
-      bhg::launch_prep( s->st, n, bhg::V1( s ), bhg::V2( s ), bhg::V3( s ), iarea, s->truearea, s->area.p, s->scal.p );
-      bhg::self_terms( s, true, false );
-      s->mesh_ready = true;
-      T.mark( s->st, 1 );
-      if ( s->method == 1 ) {  // the H-matrix build is the factorization phase
-         T.mark( s->st, 2 );
-         bhg::hm_factor( s );
-      } else {
-         bhg::assemble( s, true );
-         T.mark( s->st, 2 );
-         bhg::factor( s, precision );
-      }
-      T.mark( s->st, 3 );
-      bhg::size_vectors( s );
-      const int nrhs = ivisco ? 11 : 6;
-      bhg::launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
-      bhg::solve_device( s, nrhs );
-      T.mark( s->st, 4 );
-      bhg::compute_results( s, iarea, cg, ivisco, icov, out );
-      T.mark( s->st, 5 );
-      BH_HIP( hipStreamSynchronize( s->st ) );
-      s->stats.t_prep = T.ms( 0, 1 );
-      s->stats.t_assemble = T.ms( 1, 2 );
-      s->stats.t_factor = T.ms( 2, 3 );
-      s->stats.t_solve = T.ms( 3, 4 );
-      s->stats.t_reduce = T.ms( 4, 5 );
-      s->stats.t_total = T.ms( 0, 5 );
-      s->stats.pivoted = s->pivoted ? 1 : 0;
-      s->stats.npad = s->npad;
-      if ( s->prof.on ) s->prof.collect();
+   return guarded( s, "bhg_solve_mesh", [&] { bhg::solve_mesh( s, iarea, cg, ivisco, icov, precision, out ); } );
+}
+
+int bhg_gradient( bhg_solver* s, int iarea, double cg, int precision, bhg_result* out, double* grad ) {
+   return guarded( s, "bhg_gradient", [&] {
+      if ( s->method != 0 ) throw std::runtime_error( "the gradient needs the dense path (bhg_set_method 0)" );
+      if ( !grad ) throw std::runtime_error( "no gradient buffer" );
+      bhg::solve_mesh( s, iarea, cg, 1, 0, precision, out );
+      bhg::gradient( s, iarea, cg, grad );
    } );
 }
 

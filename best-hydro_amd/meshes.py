@@ -203,6 +203,32 @@ class Triangulation:
         return parse_best(format_best(self), self.name)
 
 
+def vertex_index(t: Triangulation):
+    """Shared vertices by exact coordinates (as COMPON matches them): (V unique vertices x 3,
+    idx NAR x 3 with t.v1[i] == V[idx[i, 0]] and so on)."""
+    allv = np.concatenate([t.v1, t.v2, t.v3])
+    V, inv = np.unique(allv, axis=0, return_inverse=True)
+    return V, inv.reshape(3, t.nar).T.copy()
+
+
+def with_vertices(t: Triangulation, V, idx, name=None) -> Triangulation:
+    """The triangulation with its shared vertices moved to V (the header kept)."""
+    V = np.asarray(V, dtype=np.float64)
+    return Triangulation(V[idx[:, 0]], V[idx[:, 1]], V[idx[:, 2]], t.truearea, t.truevol, name or t.name)
+
+
+def vertex_gradient(grad, idx, nv):
+    """Per-triangle vertex-copy gradients (..., NAR, 3 vertices, 3) summed onto the shared
+    vertices: (..., nv, 3)."""
+    g = np.asarray(grad)
+    lead = g.shape[:-3]
+    out = np.zeros(lead + (nv, 3))
+    flat = g.reshape(lead + (-1, 3))
+    for k in np.ndindex(*lead) if lead else [()]:
+        np.add.at(out[k], idx.reshape(-1), flat[k])
+    return out
+
+
 def _fortran_e(v: float, w=17, d=8) -> str:
     """Fortran Ew.d: 0.dddddddd E+xx with d significant digits."""
     if v == 0.0 or not math.isfinite(v):

@@ -108,6 +108,17 @@ int bhg_set_mesh( bhg_solver* s, int n, const double* v1, const double* v2, cons
  * MFMA + FP64 refinement). */
 int bhg_solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int precision, bhg_result* out );
 
+/* The adjoint of bhg_solve_mesh (ivisco = 1, icov = 0, dense path): one solve, then the gradient
+ * of D_t (out->dtrans), D_r (out->dr) and CHI * V (out->chiprime; [eta] = 0.602214 CHI V / M)
+ * with respect to the vertex coordinates as bhg_set_mesh took them.  grad holds 3 x 9 n doubles:
+ * grad[o * 9 n + v * 3 n + c * n + i] = d(output o) / d(coordinate c of vertex v of triangle i),
+ * o = 0 D_t, 1 D_r, 2 CHI V; the layout of v1, v2, v3 one after the other.  The adjoint system
+ * GR^T lambda = dQ/dF is solved with the factors of the forward solve, transposed in place.  The
+ * surfaces (icomp) are held fixed.  No counterpart in the reference, which has no derivatives:
+ * it differentiates the path of best/best.f:261-498 (assembly best/tgetgr.f, best/triangleint.f,
+ * deflation best/best.f:384-426, reductions best/difco.f, best/visco.f). */
+int bhg_gradient( bhg_solver* s, int iarea, double cg, int precision, bhg_result* out, double* grad );
+
 /* The last solutions (3n x nrhs, columns: 3 translations, 3 rotations, 5 strain flows). */
 int bhg_get_solution( bhg_solver* s, int nrhs, double* out, long ldo );
 int bhg_get_stats( bhg_solver* s, bhg_stats* out );

@@ -1,0 +1,207 @@
+"""The adjoint gradient of D_t, D_r and CHI V with respect to the vertex coordinates.
+
+CPU: the CPU reference gradient -- central differences (step 1e-5 A) of the oracle's full FP64
+path over every shared vertex coordinate of an 80-triangle mesh -- is pinned by what the
+mathematics fixes for any correct gradient: the outputs do not change under a translation or a
+rotation of the mesh (sum of g = 0, sum of V x g = 0), and without the area correction D_t, D_r and
+CHI V are homogeneous of degree -1, -3 and 3 in the coordinates (sum of V . g = degree * value,
+Euler's theorem).
+
+GPU: the adjoint (bhg_gradient) against that CPU reference gradient on a 320-triangle mesh, against
+central differences of the GPU solve itself on the 5,120-triangle bumpy ellipsoid and the
+two-surface mesh (per coordinate on a seeded subset of vertices and along random directions),
+against the same invariants at full size, and through torch.autograd.gradcheck.
+"""
+import numpy as np
+import pytest
+
+from best_hydro_amd import meshes as M
+
+H = 1e-5  # finite-difference step (A)
+KEYS = ("dtrans", "dr", "chiprime")
+
+
+def tiny_mesh(level=1):
+    V, F = M.icosphere(level)
+    return M.shape_mesh(M.bumpy_shape(), V, F, f"bumpy_L{level}")
+
+
+def oracle_values(oracle, tri, iarea=1):
+    r = oracle.best_run(tri.v1, tri.v2, tri.v3, tri.truearea, tri.truevol, iarea=iarea, cg=1.0, ivisco=1, icov=0)
+    return np.array([r[k] for k in KEYS])
+
+
+def fd_gradient(values, tri, V, idx, coords, h=H):
+    """Central differences of values(tri) over the shared-vertex coordinates coords ((v, c) pairs)."""
+    out = np.zeros((3, len(coords)))
+    for j, (v, c) in enumerate(coords):
+        Vp, Vm = V.copy(), V.copy()
+        Vp[v, c] += h
+        Vm[v, c] -= h
+        out[:, j] = (values(M.with_vertices(tri, Vp, idx)) - values(M.with_vertices(tri, Vm, idx))) / (2 * h)
+    return out
+
+
+def fd_directional(values, tri, V, idx, D, h=H):
+    return (values(M.with_vertices(tri, V + h * D, idx)) - values(M.with_vertices(tri, V - h * D, idx))) / (2 * h)
+
+
+def invariant_residuals(V, G, Q=None, degrees=None):
+    """Per output: |sum g| and |sum V x g| relative to sum |g| |V|, and the homogeneity defect."""
+    out = []
+    for o in range(G.shape[0]):
+        g = G[o]
+        scale = np.sum(np.abs(g)) * np.max(np.abs(V))
+        t = np.linalg.norm(g.sum(axis=0)) * np.max(np.abs(V)) / scale
+        r = np.linalg.norm(np.cross(V, g).sum(axis=0)) / scale
+        e = None
+        if Q is not None:
+            e = abs(np.sum(V * g) - degrees[o] * Q[o]) / abs(degrees[o] * Q[o])
+        out.append((t, r, e))
+    return out
+
+
+def test_oracle_fd_gradient_invariants(oracle):
+    """The CPU reference gradient satisfies the translation, rotation and scaling identities; a
+    gradient with a dropped term, a wrong sign or swapped coordinates fails them."""
+    tri = tiny_mesh(1)
+    V, idx = M.vertex_index(tri)
+    coords = [(v, c) for v in range(len(V)) for c in range(3)]
+    for iarea in (1, 0):
+        vals = lambda t: oracle_values(oracle, t, iarea)
+        G = fd_gradient(vals, tri, V, idx, coords).reshape(3, len(V), 3)
+        Q = vals(tri)
+        res = invariant_residuals(V, G, Q if iarea == 0 else None, (-1.0, -3.0, 3.0))
+        print("iarea", iarea, res)
+        for t, r, e in res:
+            assert t < 2e-8 and r < 2e-8
+            if e is not None:
+                assert e < 1e-7
+        # the identities are not trivially satisfied: one vertex's x and y swapped breaks rotation
+        Gs = G.copy()
+        Gs[:, 0, [0, 1]] = Gs[:, 0, [1, 0]]
+        assert max(r for _, r, _ in invariant_residuals(V, Gs)) > 1e-4
+
+
+# ------------------------------------------------------------------ GPU
+
+@pytest.fixture(scope="module")
+def mt_small(bh):
+    from best_hydro_amd.autograd import MeshTransport
+    return MeshTransport(tiny_mesh(2), precision=64)
+
+
+def stretch(V):
+    """A smooth deformation: anisotropic stretch plus shear (a direction with an O(1) response)."""
+    return V @ np.array([[1.0, 0.2, 0.0], [0.0, -0.5, 0.1], [0.3, 0.0, 0.7]])
+
+
+def compare(G, Q, tri, V, idx, values, coords, rng, tol_coord, tol_dir):
+    """Adjoint G against central differences of values: per coordinate (absolute, in units of the
+    output per A: an FD derivative of one vertex carries the solve's rounding divided by 2 h) and
+    along a stretch and two random directions (relative).  Prints everything, then asserts."""
+    fd = fd_gradient(values, tri, V, idx, coords)
+    ad = np.array([[G[o, v, c] for v, c in coords] for o in range(3)])
+    dirs = [("stretch", stretch(V))] + [(f"random{k}", rng.normal(size=V.shape)) for k in range(2)]
+    fails = []
+    for o in range(3):
+        e_abs = np.max(np.abs(ad[o] - fd[o])) / abs(Q[o])
+        e_rel = np.linalg.norm(ad[o] - fd[o]) / np.linalg.norm(fd[o])
+        print(f"{KEYS[o]}: {len(coords)} coordinates, max |adjoint - FD| / |Q| = {e_abs:.3e} /A, "
+              f"relative {e_rel:.3e}", flush=True)
+        if not e_abs < tol_coord:
+            fails.append((KEYS[o], "coordinates", e_abs))
+    for name, D in dirs:
+        fdd = fd_directional(values, tri, V, idx, D)
+        add = np.einsum("ovc,vc->o", G, D)
+        rel = np.abs(add - fdd) / np.abs(fdd)
+        print(f"direction {name}: adjoint {add} FD {fdd} relative {rel}", flush=True)
+        if not np.all(rel < tol_dir):
+            fails.append((name, rel))
+    assert not fails, fails
+
+
+@pytest.mark.gpu
+def test_gradient_matches_cpu_reference_gradient(bh, oracle, mt_small):
+    """320 triangles: the adjoint on the FP64 and the FP32 + refinement paths against the CPU
+    reference gradient (central differences of the oracle) on 16 seeded vertices and along three
+    directions."""
+    mt = mt_small
+    tri, V, idx = mt.tri, mt.V, mt.idx
+    print("mesh", tri.nar, "vertices", len(V), flush=True)
+    q64, G64 = mt.values_and_gradient(V)
+    ref = oracle_values(oracle, tri)
+    assert np.all(np.abs(q64 / ref - 1) < 1e-10), (q64, ref)
+    mt.precision = 32
+    try:
+        q32, G32 = mt.values_and_gradient(V)
+    finally:
+        mt.precision = 64
+    for o in range(3):
+        err32 = np.linalg.norm(G32[o] - G64[o]) / np.linalg.norm(G64[o])
+        print(KEYS[o], "FP32 path vs FP64 path", err32, flush=True)
+        assert err32 < 1e-8
+    rng = np.random.default_rng(11)
+    verts = rng.choice(len(V), 16, replace=False)
+    coords = [(int(v), c) for v in verts for c in range(3)]
+    compare(G64, q64, tri, V, idx, lambda t: oracle_values(oracle, t), coords, rng, 2e-8, 1e-6)
+
+
+def _full_size_check(bh, tri, seed, nvert, extra_verts=()):
+    from best_hydro_amd.autograd import MeshTransport
+    mt = MeshTransport(tri, precision=32)
+    V, idx = mt.V, mt.idx
+    print("mesh", tri.name, tri.nar, "vertices", len(V), "ndef", mt.icomp.max(), flush=True)
+    Q, G = mt.values_and_gradient(V)
+    print("values", Q, "stats", mt.solver.stats(), flush=True)
+    for o, (t, r, _) in enumerate(invariant_residuals(V, G)):
+        print(KEYS[o], "translation", t, "rotation", r, flush=True)
+        assert t < 1e-9 and r < 1e-8
+    # central differences of the GPU FP64 solve
+    fdm = MeshTransport(tri, solver=mt.solver, precision=64)
+    rng = np.random.default_rng(seed)
+    verts = list(rng.choice(len(V), nvert, replace=False)) + list(extra_verts)
+    coords = [(int(v), c) for v in verts for c in range(3)]
+    compare(G, Q, tri, V, idx, fdm.values_of, coords, rng, 1e-7, 1e-4)
+    return mt
+
+
+@pytest.mark.gpu
+def test_gradient_bumpy_ellipsoid_5120(bh):
+    _full_size_check(bh, M.bumpy_ellipsoid_mesh(), seed=5, nvert=8)
+
+
+@pytest.mark.gpu
+def test_gradient_two_surfaces(bh):
+    tri = M.two_surface_mesh()
+    V, idx = M.vertex_index(tri)
+    small = np.unique(idx[5120:])
+    _full_size_check(bh, tri, seed=6, nvert=6, extra_verts=small[:2])
+
+
+@pytest.mark.gpu
+def test_gradient_scaling_identity(bh, mt_small):
+    """Without the area correction: sum V . grad = degree * value (degrees -1, -3, 3)."""
+    from best_hydro_amd.autograd import MeshTransport
+    mt = MeshTransport(mt_small.tri, solver=mt_small.solver, iarea=0, precision=64)
+    Q, G = mt.values_and_gradient(mt.V)
+    for o, (t, r, e) in enumerate(invariant_residuals(mt.V, G, Q, (-1.0, -3.0, 3.0))):
+        print(KEYS[o], t, r, e, flush=True)
+        assert t < 1e-9 and r < 1e-9 and e < 1e-9
+
+
+@pytest.mark.gpu
+def test_autograd_gradcheck(bh, mt_small):
+    import torch
+    from best_hydro_amd.autograd import MeshTransport, transport
+    mt = MeshTransport(tiny_mesh(1), solver=mt_small.solver, precision=64, wm=14000.0)
+    V = torch.tensor(mt.V, dtype=torch.float64, requires_grad=True)
+    q0 = torch.tensor(mt.values(mt.V), dtype=torch.float64)
+    f = lambda X: transport(X, mt) / q0
+    assert torch.autograd.gradcheck(f, (V,), eps=1e-6, atol=1e-7, rtol=1e-5)
+    # backward of a scalar combination equals the combination of the rows
+    y = (transport(V, mt) * torch.tensor([1.0, 2.0, 3.0, 4.0], dtype=torch.float64) / q0).sum()
+    y.backward()
+    _, G = mt.values_and_gradient(mt.V)
+    want = np.einsum("o,ovc->vc", np.array([1.0, 2.0, 3.0, 4.0]) / q0.numpy(), G)
+    assert np.allclose(V.grad.numpy(), want, rtol=1e-12, atol=0)
