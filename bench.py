@@ -11,7 +11,9 @@ mesh, the errors of both paths against the oracle (the reference's dense FP64 pa
 with LAPACK), the roofline of the dominant kernel class (HIP events around every launch of the
 class during the timed steps), the oracle's CPU rate on this host, and the H-matrix path on the
 81,920-triangle sphere (hmatrix_c5: solves/s, build and GMRES times, and the HBM roofline of the
-H-matrix product).
+H-matrix product), the series of eight meshes for the 1/N extrapolation solved by one captured
+graph launch (series_c3), and the adjoint gradient of D_t, D_r and CHI*V with respect to the
+vertex coordinates on the 5,120-triangle mesh and the two-surface mesh (adjoint_c4).
 """
 import argparse
 import importlib.util
@@ -19,6 +21,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
@@ -90,6 +94,67 @@ def hmatrix_c5(bh, M, steps):
                          "bytes_per_product": nbytes}}
 
 
+def series_c3(bh, M, steps):
+    """The 1/N series: the bumpy ellipsoid remeshed at 8 uneven sizes (2,500 ... 7,000 triangles),
+    all meshes resident; the autotuner picks the tiles and captures the schedule; one step is one
+    launch of the captured graph (every mesh's FP32-LU + FP64-IR solve) plus the per-mesh checks
+    and reductions; then the linear fits in 1/N."""
+    tris = M.series_meshes()
+    sizes = [t.nar for t in tris]
+    ser = bh.Series(tris)
+    t0 = time.perf_counter()
+    tune = ser.autotune(ivisco=1)
+    t_tune = time.perf_counter() - t0
+    ser.solve(ivisco=1)  # warm
+    t0 = time.perf_counter()
+    gms = []
+    for _ in range(steps):
+        res, ms = ser.solve(ivisco=1)
+        gms.append(ms)
+    el = (time.perf_counter() - t0) / steps
+    fits = {}
+    for key in ("dtrans", "dr", "chiprime"):
+        a, b = bh.extrapolate_1n(sizes, [r[key] for r in res])
+        fits[key] = {"intercept": a, "slope": b}
+    pivoted = [ser.stats(k)["pivoted"] for k in range(len(tris))]
+    ser.close()
+    return {"workload": "bumpy ellipsoid remeshed by random-point Delaunay at N = " + ", ".join(map(str, sizes)),
+            "value": round(len(tris) / el, 4), "unit": "mesh solves/s", "s_per_series": round(el, 4),
+            "graph_ms": round(min(gms), 3), "steps": steps, "autotune_s": round(t_tune, 2),
+            "tuning": tune, "pivoted": pivoted, "fit_1n": fits}
+
+
+def adjoint_c4(bh, M, steps):
+    """The adjoint: one solve (FP32 LU + FP64 IR) and the gradient of D_t, D_r and CHI*V with
+    respect to every vertex coordinate, on the 5,120-triangle bumpy ellipsoid and on the
+    two-surface mesh (that ellipsoid plus an 80-triangle 2 A sphere 12 A away, two deflation
+    terms).  The pair term of the gradient (k_adj_pairs) is timed with HIP events."""
+    out = {}
+    for name, tri in (("bumpy_ellipsoid_5120", M.bumpy_ellipsoid_mesh()), ("two_surface_5200", M.two_surface_mesh())):
+        s = bh.Solver(0)
+        s.set_mesh(tri)
+        s.gradient(precision=32)  # warm
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.solve_mesh(precision=32)
+        t_solve = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r, g = s.gradient(precision=32)
+        t_all = (time.perf_counter() - t0) / steps
+        st = s.stats()
+        V, idx = M.vertex_index(tri)
+        gv = M.vertex_gradient(g, idx, len(V))
+        s.close()
+        out[name] = {"triangles": tri.nar, "vertices": len(V), "coordinates": 3 * len(V),
+                     "solve_ms": round(1e3 * t_solve, 2), "solve_and_gradient_ms": round(1e3 * t_all, 2),
+                     "gradient_ms": round(1e3 * (t_all - t_solve), 2),
+                     "adjoint_pairs_ms": round(st["t_adjoint_pairs"], 2),
+                     "grad_norm": {"dtrans": float(np.linalg.norm(gv[0])), "dr": float(np.linalg.norm(gv[1])),
+                                   "chiprime": float(np.linalg.norm(gv[2]))}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +162,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline and error report")
     ap.add_argument("--no-hmatrix", action="store_true", help="skip the H-matrix path's 81,920-triangle case")
+    ap.add_argument("--no-series", action="store_true", help="skip the 8-mesh series")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve does not shard across "
@@ -165,7 +232,7 @@ def main():
                    "parallelism": "single GPU"},
         "fp64_lu": {"value": round(1e3 / ms64, 4), "unit": "solves/s", "ms_per_step": round(ms64, 3)},
         "ratio_mixed_over_fp64": round(ms64 / ms32, 3),
-        "phases_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("t_")},
+        "phases_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("t_") and not k.startswith("t_adjoint")},
         "ir_iterations": st32["ir_iterations"],
         "roofline": roofline,
     }
@@ -173,7 +240,6 @@ def main():
         ref, el = cpu_baseline(tri, cores)
         def err(a, b):
             return abs(a - b) / abs(b)
-        import numpy as np
         errors = {}
         for name, r in (("mixed", r32), ("fp64", r64)):
             errors[name] = {"dtrans": err(r["dtrans"], ref["dtrans"]), "dr": err(r["dr"], ref["dr"]),
@@ -185,6 +251,10 @@ def main():
                                          f"OpenMP {cores} threads; LAPACK dgetrf/dgetrs via SciPy OpenBLAS; C DIFCO/VISCO): "
                                          f"{el:.2f} s"}
     s.close()
+    if not args.no_series:
+        out["series_c3"] = series_c3(bh, M, max(1, min(args.steps, 5)))
+    if not args.no_adjoint:
+        out["adjoint_c4"] = adjoint_c4(bh, M, max(1, min(args.steps, 5)))
     if not args.no_hmatrix:
         out["hmatrix_c5"] = hmatrix_c5(bh, M, max(1, min(args.steps, 3)))
     print(json.dumps(out), flush=True)

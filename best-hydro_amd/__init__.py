@@ -58,7 +58,8 @@ class BhResult(C.Structure):
 class BhStats(C.Structure):
     _fields_ = [("t_prep", C.c_double), ("t_assemble", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_reduce", C.c_double), ("t_total", C.c_double), ("ir_iterations", C.c_int),
-                ("ir_correction", C.c_double), ("pivoted", C.c_int), ("npad", C.c_int)]
+                ("ir_correction", C.c_double), ("pivoted", C.c_int), ("npad", C.c_int),
+                ("t_adjoint_solve", C.c_double), ("t_adjoint_pairs", C.c_double), ("t_adjoint_tri", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -690,6 +690,8 @@ void gradient( bhg_solver* s, int iarea, double cg, double* grad ) {
    // adjoint right-hand sides and solves with the transposed factors
    s->adj_y.ensure( np * K * O );
    s->adj_lam.ensure( np * K * O );
+   auto& T = s->timer;
+   T.mark( st, 0 );
    fill( st, s->adj_y.p, np * K * O, 0.0 );
    launch_adj_rhs( st, n, s->xc.p, s->area.p, s->adj_ghat.p, s->adj_y.p, np );
    transpose_factors( s );
@@ -705,6 +707,7 @@ void gradient( bhg_solver* s, int iarea, double cg, double* grad ) {
    }
    transpose_factors( s );
    BH_HIP( hipMemcpyAsync( s->x.p, s->adj_F.p, np * K * sizeof( double ), hipMemcpyDeviceToDevice, st ) );
+   T.mark( st, 1 );
    // pair terms (source strips sized for at least 1024 workgroups) and per-triangle terms
    const int nb = ( n + 255 ) / 256;
    const int strip = std::max( 8, std::min( 128, (int)( ( (long)n * nb + 1023 ) / 1024 ) ) );
@@ -716,8 +719,10 @@ void gradient( bhg_solver* s, int iarea, double cg, double* grad ) {
    s->adj_tri.ensure( n );
    launch_adj_pairs( st, n, strip, s->cg, s->src.p, s->xc.p, s->adj_lam.p, s->adj_F.p, np, s->adj_gsrc_part.p, s->adj_gcol_part.p,
                      s->adj_gsrc.p, s->adj_gcol.p );
+   T.mark( st, 2 );
    launch_adj_tri( st, n, V1( s ), V2( s ), V3( s ), s->scal.p, s->cg, s->adj_lam.p, s->adj_F.p, np, s->adj_ghat.p,
                    s->adj_tri.p );
+   T.mark( st, 3 );
    // to the host: the chain rule through the deflation, the area factor and the centroid shift
    std::vector<double> gsrc( (size_t)n * O * 10 ), gcol( (size_t)n * O * 3 ), lam( (size_t)n3 * K * O ), F( (size_t)n3 * K );
    std::vector<AdjTri> tri( n );
@@ -736,6 +741,9 @@ void gradient( bhg_solver* s, int iarea, double cg, double* grad ) {
    BH_HIP( hipMemcpyAsync( gdef.data(), s->gdef.p, ndef * sizeof( double ), hipMemcpyDeviceToHost, st ) );
    BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, st ) );
    BH_HIP( hipStreamSynchronize( st ) );
+   s->stats.t_adjoint_solve = T.ms( 0, 1 );
+   s->stats.t_adjoint_pairs = T.ms( 1, 2 );
+   s->stats.t_adjoint_tri = T.ms( 2, 3 );
    const double afac = scal[ 0 ], tot = scal[ 1 ], diagm = scal[ 5 ];
    auto Lm = [&]( int o, int k, int i, int c ) { return lam[ ( (size_t)o * K + k ) * n3 + 3 * (size_t)i + c ]; };
    auto Fm = [&]( int k, int i, int c ) { return F[ (size_t)k * n3 + 3 * (size_t)i + c ]; };

@@ -29,6 +29,7 @@ module best_hip
       integer(c_int) :: ir_iterations
       real(c_double) :: ir_correction
       integer(c_int) :: pivoted, npad
+      real(c_double) :: t_adjoint_solve, t_adjoint_pairs, t_adjoint_tri
    end type bhg_stats
 
    interface

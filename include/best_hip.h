@@ -86,6 +86,8 @@ typedef struct bhg_stats {
    double ir_correction; /* last max_j |dx_j| / |x_j| */
    int pivoted;          /* 1 if the factorization needed row exchanges */
    int npad;             /* padded order of the matrix */
+   double t_adjoint_solve, t_adjoint_pairs, t_adjoint_tri; /* bhg_gradient: the transposed solves
+                                                              (3 x 11 RHS), the pair term, the rest */
 } bhg_stats;
 
 bhg_solver* bhg_create( int device );
