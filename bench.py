@@ -87,7 +87,9 @@ def hmatrix_c5(bh, M, steps):
             "build_s": round(st["t_tree"] + st["t_fill"] + st["t_pre"], 3), "gmres_s": round(st["t_solve"], 3),
             "gmres_iterations": [st["iters_min"], st["iters_max"]], "relres": st["relres"],
             "h_gb": round((st["bytes_dense"] + st["bytes_lowrank"]) / 1e9, 2), "rank_max": st["kmax"],
-            "rank_mean": round(st["kmean"], 1), "dtrans": r["dtrans"], "dr": r["dr"],
+            "rank_mean": round(st["kmean"], 1), "rank_mean_before_recompression": round(st["kmean_aca"], 1),
+            "recompression_s": round(st["t_trunc"], 3), "blocks_not_recompressed": st["n_trunc_failed"],
+            "dtrans": r["dtrans"], "dr": r["dr"],
             "roofline": {"kernel": "H-matrix product (k_hm_vtx + k_hm_rows + sums)", "bound": "hbm",
                          "achieved": round(gbs, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
                          "frac": round(gbs / PEAKS["hbm_gbs"], 4), "avg_product_ms": round(ms, 4),

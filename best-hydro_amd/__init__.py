@@ -71,7 +71,8 @@ class BhHmStats(C.Structure):
                 ("bytes_dense", C.c_double), ("bytes_lowrank", C.c_double), ("kmax", C.c_int), ("kmean", C.c_double),
                 ("kernel_blocks", C.c_double), ("t_tree", C.c_double), ("t_fill", C.c_double), ("t_pre", C.c_double),
                 ("iters_min", C.c_int), ("iters_max", C.c_int), ("steps", C.c_int), ("relres", C.c_double),
-                ("t_solve", C.c_double)]
+                ("t_solve", C.c_double), ("n_trunc_failed", C.c_long), ("kmean_aca", C.c_double),
+                ("t_trunc", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -166,6 +167,8 @@ def lib():
     L.bhg_hm_time_product.restype = i
     L.bhg_hm_apply.argtypes = [vp, i, _dp, lg, _dp, lg]
     L.bhg_hm_apply.restype = i
+    L.bhg_hm_blocks.argtypes = [vp, C.c_void_p, lg, C.c_void_p, C.POINTER(lg)]
+    L.bhg_hm_blocks.restype = i
     L.bhg_gradient.argtypes = [vp, i, d, i, C.POINTER(BhResult), _dp]
     L.bhg_gradient.restype = i
     _LIB = L
@@ -307,6 +310,18 @@ class Solver:
         self._check(self.L.bhg_hm_apply(self.h, nrhs, x.reshape(-1, order="F").copy(), 3 * self.n, y, 3 * self.n),
                     "hm_apply")
         return y.reshape(x.shape, order="F")
+
+    def hm_blocks(self):
+        """The leaf blocks of the built H-matrix as an (nblocks, 6) int64 array (1 if low-rank, rank,
+        first row, rows, first column, columns; unknowns in tree order) and the tree order of the
+        triangles (BEST's triangle, from 0, at each tree position)."""
+        cnt = C.c_long()
+        self._check(self.L.bhg_hm_blocks(self.h, None, 0, None, C.byref(cnt)), "hm_blocks")
+        out = np.zeros((cnt.value, 6), dtype=np.int64)
+        perm = np.zeros(self.n, dtype=np.int32)
+        self._check(self.L.bhg_hm_blocks(self.h, out.ctypes.data, cnt.value, perm.ctypes.data, C.byref(cnt)),
+                    "hm_blocks")
+        return out, perm
 
     # -- the dense path piece by piece (the besolve module's twins)
     def load_prepared(self, n, v1, v2, v3, area, afac, cg=1.0):

@@ -19,6 +19,7 @@ struct HmOptions {
    long dense_small = 1024;    // admissible blocks of at most this many triangle pairs are kept dense
    int kcap = 192;             // rank at which a cross approximation gives up (the block is then kept dense)
    int single = 0;             // 1: blocks stored in FP32 (arithmetic in FP64)
+   int trunc = 1;              // 1: approximations recompressed to eps by QR and SVD, as the reference does
    double tol = 0.0;           // GMRES relative residual; 0: aca_factor * eps
    int restart = 30;           // GMRES restart length
    int maxsteps = 500;         // GMRES steps (batched products) per solve
@@ -38,6 +39,9 @@ struct HmStats {
    double t_tree = 0, t_fill = 0, t_pre = 0;   // seconds
    int iters_min = 0, iters_max = 0, steps = 0;
    double relres = 0, t_solve = 0;
+   long n_trunc_failed = 0;    // approximations left as they were (dependent columns, or rank above the limit)
+   double kmean_aca = 0;       // mean rank before the recompression
+   double t_trunc = 0;         // seconds in the recompression (part of t_fill)
 };
 
 // The geometry: vertices (v1, v2, v3 as BEST's NAR x 3 column-major arrays, one after the other)
@@ -78,6 +82,11 @@ void hm_solve( HMatrix& h, hipStream_t st, int nrhs, const double* d_b, long ldb
 // Mean duration (ms, HIP events) of the product with p columns of device-resident vectors over
 // reps repetitions; *bytes = the blocks' storage plus X and Y (the algorithmic HBM bytes).
 double hm_time_product( HMatrix& h, hipStream_t st, int p, int reps, double* bytes );
+
+// The leaf blocks, 6 longs each (1 if low-rank else 0, rank, r0, nr, c0, nc; unknowns in tree
+// order), at most cap of them into out; perm (n ints) = the triangle at each tree position.
+// Returns the number of blocks.
+long hm_blocks( const HMatrix& h, long* out, long cap, int* perm );
 
 // y = (GR + deflation) x on the device, BEST's order (for checks).
 void hm_apply( HMatrix& h, hipStream_t st, int nrhs, const double* d_x, long ldx, double* d_y, long ldy );

@@ -8,6 +8,9 @@
 //                   low-rank leaves, pairs of leaf clusters dense leaves (host)
 //   k_hm_aca        hmatrix/src/hmatrix.cpp:62-251: adaptive cross approximation with partial
 //                   pivoting, one workgroup per block, a whole batch of blocks per launch
+//   k_hm_trunc      hmatrix/src/mat.cpp:129-160: each approximation recompressed to eps (QR of
+//                   both factors, SVD of the small core, singular values above eps s_max kept),
+//                   one workgroup per block; k_hm_trunc_apply forms the new factors
 //   k_hm_dense      dense leaves (hmatrix/src/kernel.cpp:251-275), one workgroup per block
 //   k_hm_vtx, k_hm_rows   the product Y = H X (hmatrix/src/hmatrix.cpp:441-471): V^T X per
 //                   low-rank block, then one workgroup per leaf row cluster sums its blocks in a
@@ -16,10 +19,11 @@
 //                   the right-hand sides; Krylov vectors and products on the device, the small
 //                   Hessenberg algebra on the host
 //
-// Differences from the reference, by design: the low-rank blocks are kept as the cross
-// approximation leaves them at aca_factor * eps (the reference recompresses them to eps by QR and
-// SVD: its ranks are lower, the accuracy the same or better here); small admissible blocks and
-// blocks whose approximation reaches kcap are stored dense; the deflation is applied as its own
+// Differences from the reference, by design: the QR factors of the recompression come from
+// Cholesky factors of the Gram matrices (one parallel pass over the factors; the kept ranks agree
+// with the exact blocks' numerical ranks at eps); small admissible blocks and blocks whose
+// approximation reaches kcap are stored dense (the reference compresses them by pivoted QR and
+// SVD); the deflation is applied as its own
 // rank-ndef term of the product instead of being appended to every block; the preconditioner is
 // block Jacobi on the dense diagonal leaves (the reference factors a coarse H-LU).  GMRES stops
 // at the same relative residual, so the solution agrees to that tolerance.
@@ -530,6 +534,358 @@ __global__ void k_hm_compact( const AcaJob* __restrict__ jobs, const int* __rest
    for ( long e = threadIdx.x; e < N * k; e += blockDim.x ) {
       const long c = e / k, l = e % k;
       Vo[ e ] = (E)J.V[ l * N + c ];
+   }
+}
+
+// ------------------------------------------------------------------ recompression
+
+// The reference recompresses each cross approximation U V^T (rank k, accurate to aca_factor *
+// eps) to eps: U = Qu Ru and V = Qv Rv by Householder QR, the SVD of Ru Rv^T, and the singular
+// values above eps * s_max kept (hmatrix/src/mat.cpp:129-160, kept_rank :22-32).  Here the QR
+// factors come from the Gram matrices instead (Cholesky QR: one streaming pass over U and V, in
+// parallel over the rows, where Householder QR makes k dependent passes): G = U^T U with the
+// columns scaled to unit norm, G = R^T R, so U = Q R D with Q = U D^-1 R^-1.  U V^T = Qu M Qv^T
+// holds for the computed factors whatever their rounding, M = (Ru Du)(Rv Dv)^T; one-sided Jacobi
+// on M's columns gives M Z = B with orthogonal columns (the singular values are their norms), and
+// the kept columns give U' = U (Du^-1 Ru^-1 B_r), V' = V (Dv^-1 Rv^-1 Z_r).  A Gram matrix whose
+// scaled pivot falls below 1e-12 (columns dependent to 6 digits) is not factored: that block keeps
+// its cross approximation unchanged.
+//
+// The blocks are recompressed in four classes by rank, each with a workgroup and an LDS size of
+// its own (kp <= 32: 128 threads, 16 KiB; <= 64: 256, 64 KiB; <= 96: 512, 144 KiB; <= 128: 512,
+// 128 KiB with the rotations Z in global memory), so that small blocks run several to a CU.
+constexpr int kTrMaxK = 128;  // ranks up to this are recompressed
+constexpr int kTrRows = 32;   // rows of U or V staged per step of the Gram products
+constexpr int kTrApply = 256; // rows per workgroup of k_hm_trunc_apply
+
+struct TruncJob {
+   const double* U;  // M x k, leading dimension M (the cross approximation's work space)
+   const double* V;  // N x k, leading dimension N
+   int M, N, k, kp;  // kp = k rounded up to a multiple of 4
+   double* G;        // 2 kp x kp: the Gram matrices of U and V, then their Cholesky factors
+   double* Z;        // kp x kp: the right rotations of the SVD of M (used when they do not fit LDS)
+   double* C;        // 2 kp x kp: Cu and Cv, U' = U Cu, V' = V Cv (r columns each)
+   double* d;        // 2 kp: the column norms of U and V
+};
+
+// Gram matrix A^T A (A: R x k, column-major) into G (kp x kp, both halves), fixed summation order.
+// Threads own 4 x 4 tiles of the upper triangle (at most Q per thread); with fewer tiles than
+// threads, groups of threads take interleaved rows of each staged chunk and their sums are added
+// in group order.
+template <int NT, int Q>
+__device__ void tr_gram( const double* __restrict__ A, int R, int k, int kp, double* __restrict__ G, double* S ) {
+   const int tid = threadIdx.x;
+   const int nt = kp / 4, npairs = nt * ( nt + 1 ) / 2;
+   const int stride = npairs >= NT ? NT : npairs;
+   const int ngrp = npairs >= NT ? 1 : NT / npairs;
+   const int grp = tid / stride;
+   const bool active = grp < ngrp;
+   int ti[ Q ], tj[ Q ];
+   bool has[ Q ];
+#pragma unroll
+   for ( int q = 0; q < Q; ++q ) {
+      int p = tid % stride + stride * q;
+      has[ q ] = active && p < npairs && ( q == 0 || npairs >= NT );
+      int a = 0;
+      while ( has[ q ] && p >= nt - a ) p -= nt - a, ++a;
+      ti[ q ] = a, tj[ q ] = a + p;
+   }
+   double acc[ Q ][ 16 ];
+#pragma unroll
+   for ( int q = 0; q < Q; ++q )
+#pragma unroll
+      for ( int e = 0; e < 16; ++e ) acc[ q ][ e ] = 0.0;
+   for ( int r0 = 0; r0 < R; r0 += kTrRows ) {
+      __syncthreads();
+      for ( int e = tid; e < kTrRows * kp; e += NT ) {
+         const int r = e % kTrRows, c = e / kTrRows;
+         S[ r * kp + c ] = ( c < k && r0 + r < R ) ? A[ (long)c * R + r0 + r ] : 0.0;
+      }
+      __syncthreads();
+      if ( !active ) continue;
+      for ( int r = grp; r < kTrRows; r += ngrp ) {
+         const double* row = S + r * kp;
+#pragma unroll
+         for ( int q = 0; q < Q; ++q ) {
+            if ( !has[ q ] ) continue;
+            double a[ 4 ], b[ 4 ];
+#pragma unroll
+            for ( int x = 0; x < 4; ++x ) a[ x ] = row[ 4 * ti[ q ] + x ], b[ x ] = row[ 4 * tj[ q ] + x ];
+#pragma unroll
+            for ( int x = 0; x < 4; ++x )
+#pragma unroll
+               for ( int y = 0; y < 4; ++y ) acc[ q ][ 4 * x + y ] += a[ x ] * b[ y ];
+         }
+      }
+   }
+   __syncthreads();
+   if ( ngrp > 1 ) {
+      // per-group sums through LDS, added in group order
+      if ( active )
+#pragma unroll
+         for ( int e = 0; e < 16; ++e ) S[ ( grp * npairs + tid % stride ) * 16 + e ] = acc[ 0 ][ e ];
+      __syncthreads();
+      for ( int o = tid; o < npairs * 16; o += NT ) {
+         const int p0 = o / 16, e = o % 16;
+         double s = 0.0;
+         for ( int g = 0; g < ngrp; ++g ) s += S[ ( g * npairs + p0 ) * 16 + e ];
+         int p = p0, a = 0;
+         while ( p >= nt - a ) p -= nt - a, ++a;
+         const int i = 4 * a + e / 4, j = 4 * ( a + p ) + e % 4;
+         G[ (long)j * kp + i ] = s;
+         G[ (long)i * kp + j ] = s;
+      }
+   } else {
+#pragma unroll
+      for ( int q = 0; q < Q; ++q ) {
+         if ( !has[ q ] ) continue;
+#pragma unroll
+         for ( int e = 0; e < 16; ++e ) {
+            const int i = 4 * ti[ q ] + e / 4, j = 4 * tj[ q ] + e % 4;
+            G[ (long)j * kp + i ] = acc[ q ][ e ];
+            G[ (long)i * kp + j ] = acc[ q ][ e ];
+         }
+      }
+   }
+   __syncthreads();
+}
+
+// Cholesky factor of the column-scaled Gram matrix G (kp x kp, in place: the upper triangle
+// becomes R, the rest zero) and the column norms d.  Returns false if a scaled pivot is below
+// 1e-12 or a column is zero (the result is then unused).
+template <int NT>
+__device__ bool tr_chol( double* __restrict__ G, double* __restrict__ d, int k, int kp, double* S, double* flag ) {
+   const int tid = threadIdx.x;
+   for ( int e = tid; e < kp * kp; e += NT ) S[ e ] = G[ e ];
+   __syncthreads();
+   for ( int l = tid; l < kp; l += NT ) d[ l ] = l < k ? sqrt( S[ l * kp + l ] ) : 1.0;
+   if ( tid == 0 ) flag[ 0 ] = 0.0;
+   __syncthreads();
+   for ( int e = tid; e < k * k; e += NT ) {
+      const int i = e % k, c = e / k;
+      const double di = d[ i ], dc = d[ c ];
+      if ( !( di > 0.0 ) || !( dc > 0.0 ) || !isfinite( di ) || !isfinite( dc ) ) flag[ 0 ] = 1.0;
+      else S[ c * kp + i ] = S[ c * kp + i ] / ( di * dc );
+   }
+   __syncthreads();
+   if ( flag[ 0 ] != 0.0 ) return false;
+   for ( int j = 0; j < k; ++j ) {
+      const double piv = S[ j * kp + j ];
+      if ( !( piv >= 1e-12 ) ) return false;  // uniform: every thread reads the same LDS value
+      const double rjj = sqrt( piv );
+      __syncthreads();
+      for ( int c = j + 1 + tid; c < k; c += NT ) S[ c * kp + j ] = S[ c * kp + j ] / rjj;
+      if ( tid == 0 ) S[ j * kp + j ] = rjj;
+      __syncthreads();
+      const int w = k - j - 1;
+      for ( int e = tid; e < w * w; e += NT ) {
+         const int i = j + 1 + e % w, c = j + 1 + e / w;
+         if ( i <= c ) S[ c * kp + i ] -= S[ i * kp + j ] * S[ c * kp + j ];
+      }
+      __syncthreads();
+   }
+   for ( int e = tid; e < kp * kp; e += NT ) {
+      const int i = e % kp, c = e / kp;
+      G[ e ] = ( i <= c && c < k ) ? S[ e ] : 0.0;
+   }
+   __syncthreads();
+   return true;
+}
+
+// One-sided Jacobi on the k columns of M (LDS, column-major, leading dimension kp), the rotations
+// accumulated in Z (LDS or global memory): cyclic round-robin pairs, all pairs of a round at once,
+// each on a group of L lanes (a power of two, L <= 64, pairs x L <= the workgroup; the group's
+// sums are its first lane's, so every lane applies the same rotation), at most 30 sweeps, a
+// rotation wherever |m_a . m_b| > 1e-13 |m_a| |m_b|.
+template <int NT>
+__device__ void tr_jacobi( double* M, double* Z, int k, int kp, double* flag ) {
+   const int tid = threadIdx.x;
+   const int nk = k + ( k & 1 ), np = nk / 2;
+   int L = 64;
+   while ( L > 1 && np * L > NT ) L >>= 1;
+   const int grp = tid / L, gl = tid % L, nslot = NT / L;
+   for ( int sweep = 0; sweep < 30 && nk > 1; ++sweep ) {
+      bool rotated = false;
+      for ( int rd = 0; rd < nk - 1; ++rd ) {
+         for ( int s = grp; s < np; s += nslot ) {
+            int a, b;
+            if ( s == 0 ) a = rd, b = nk - 1;
+            else a = ( rd + s ) % ( nk - 1 ), b = ( rd - s + nk - 1 ) % ( nk - 1 );
+            if ( a > b ) {
+               const int t = a;
+               a = b, b = t;
+            }
+            if ( b >= k ) continue;  // the whole group: its lanes share the pair
+            double* Ma = M + a * kp;
+            double* Mb = M + b * kp;
+            double al = 0.0, be = 0.0, ga = 0.0;
+            for ( int i = gl; i < k; i += L ) {
+               const double x = Ma[ i ], y = Mb[ i ];
+               al += x * x, be += y * y, ga += x * y;
+            }
+            for ( int o = L >> 1; o > 0; o >>= 1 ) {
+               al += __shfl_xor( al, o );
+               be += __shfl_xor( be, o );
+               ga += __shfl_xor( ga, o );
+            }
+            al = __shfl( al, 0, L ), be = __shfl( be, 0, L ), ga = __shfl( ga, 0, L );
+            if ( al == 0.0 || be == 0.0 || !( fabs( ga ) > 1e-13 * sqrt( al * be ) ) ) continue;
+            const double zeta = ( be - al ) / ( 2.0 * ga );
+            const double t = ( zeta >= 0.0 ? 1.0 : -1.0 ) / ( fabs( zeta ) + sqrt( 1.0 + zeta * zeta ) );
+            const double c = 1.0 / sqrt( 1.0 + t * t ), sn = c * t;
+            double* Za = Z + (long)a * kp;
+            double* Zb = Z + (long)b * kp;
+            for ( int i = gl; i < k; i += L ) {
+               const double x = Ma[ i ], y = Mb[ i ];
+               Ma[ i ] = c * x - sn * y;
+               Mb[ i ] = sn * x + c * y;
+               const double zx = Za[ i ], zy = Zb[ i ];
+               Za[ i ] = c * zx - sn * zy;
+               Zb[ i ] = sn * zx + c * zy;
+            }
+            rotated = true;
+         }
+         __syncthreads();
+      }
+      if ( tid == 0 ) flag[ 0 ] = 0.0;
+      __syncthreads();
+      if ( rotated ) flag[ 0 ] = 1.0;
+      __syncthreads();
+      const bool more = flag[ 0 ] != 0.0;
+      __syncthreads();
+      if ( !more ) break;
+   }
+}
+
+// One workgroup per block (ranks with kp <= KP): Gram matrices, Cholesky factors, the SVD of M by
+// one-sided Jacobi, the kept rank, and the coefficient matrices Cu, Cv.  rank[j] = the new rank
+// (the old one if the block is not recompressed, with Cu = Cv = I), failed[j] = 1 in that case.
+template <int NT, int KP>
+__global__ void __launch_bounds__( NT ) k_hm_trunc( const TruncJob* __restrict__ jobs, double eps, int* __restrict__ rank,
+                                                    int* __restrict__ failed ) {
+   constexpr bool kZLds = KP <= 96;
+   constexpr int kLds = ( kZLds ? 2 : 1 ) * KP * KP;
+   constexpr int kTiles = ( KP / 4 ) * ( KP / 4 + 1 ) / 2;
+   static_assert( kTrRows * KP <= kLds && 16 * NT <= kLds, "LDS too small for the Gram products" );
+   __shared__ __attribute__( ( aligned( 16 ) ) ) double S[ kLds ];
+   __shared__ double sig[ KP ];
+   __shared__ int ord[ KP ];
+   __shared__ double flag[ 4 ];
+   const TruncJob J = jobs[ blockIdx.x ];
+   const int tid = threadIdx.x, k = J.k, kp = J.kp;
+   double* Gu = J.G;
+   double* Gv = J.G + (long)kp * kp;
+   double* Cu = J.C;
+   double* Cv = J.C + (long)kp * kp;
+   tr_gram<NT, ( kTiles + NT - 1 ) / NT>( J.U, J.M, k, kp, Gu, S );
+   tr_gram<NT, ( kTiles + NT - 1 ) / NT>( J.V, J.N, k, kp, Gv, S );
+   bool ok = tr_chol<NT>( Gu, J.d, k, kp, S, flag );
+   if ( ok ) ok = tr_chol<NT>( Gv, J.d + kp, k, kp, S, flag );
+   if ( !ok ) {
+      for ( int e = tid; e < kp * kp; e += NT ) {
+         const double v = ( e % kp == e / kp ) ? 1.0 : 0.0;
+         Cu[ e ] = v, Cv[ e ] = v;
+      }
+      if ( tid == 0 ) rank[ blockIdx.x ] = k, failed[ blockIdx.x ] = 1;
+      return;
+   }
+   const double* du = J.d;
+   const double* dv = J.d + kp;
+   double* M = S;
+   double* Z = kZLds ? S + kp * kp : J.Z;
+   // M = (Ru Du)(Rv Dv)^T (column-major); Z = I
+   for ( int e = tid; e < kp * kp; e += NT ) {
+      const int i = e % kp, j = e / kp;
+      double s = 0.0;
+      if ( i < k && j < k )
+         for ( int l = max( i, j ); l < k; ++l ) s += ( Gu[ (long)l * kp + i ] * du[ l ] ) * ( Gv[ (long)l * kp + j ] * dv[ l ] );
+      M[ e ] = s;
+      Z[ e ] = ( i == j ) ? 1.0 : 0.0;
+   }
+   __syncthreads();
+   tr_jacobi<NT>( M, Z, k, kp, flag );
+   // singular values, their order (decreasing, ties by index) and the kept rank
+   for ( int j = tid; j < k; j += NT ) {
+      double s = 0.0;
+      for ( int i = 0; i < k; ++i ) s += M[ j * kp + i ] * M[ j * kp + i ];
+      sig[ j ] = sqrt( s );
+   }
+   __syncthreads();
+   for ( int j = tid; j < k; j += NT ) {
+      int pos = 0;
+      for ( int i = 0; i < k; ++i ) pos += ( sig[ i ] > sig[ j ] || ( sig[ i ] == sig[ j ] && i < j ) ) ? 1 : 0;
+      ord[ pos ] = j;
+   }
+   __syncthreads();
+   const double smax = sig[ ord[ 0 ] ];
+   int r = 0;
+   if ( smax > 0.0 )
+      for ( int j = 0; j < k; ++j ) r += sig[ j ] > eps * smax ? 1 : 0;
+   // Cu(:, c) = Du^-1 Ru^-1 M(:, ord[c]) and Cv(:, c) = Dv^-1 Rv^-1 Z(:, ord[c]), c < r: back
+   // substitution in place on the kept columns, row i of all 2 r columns at a time
+   for ( int i = k - 1; i >= 0; --i ) {
+      for ( int c = tid; c < 2 * r; c += NT ) {
+         const bool isv = c >= r;
+         double* x = ( isv ? Z : M ) + (long)ord[ isv ? c - r : c ] * kp;
+         x[ i ] = x[ i ] / ( isv ? Gv : Gu )[ (long)i * kp + i ];
+      }
+      __syncthreads();
+      for ( int e = tid; e < 2 * r * i; e += NT ) {
+         const int l = e % i, c = e / i;
+         const bool isv = c >= r;
+         double* x = ( isv ? Z : M ) + (long)ord[ isv ? c - r : c ] * kp;
+         x[ l ] -= ( isv ? Gv : Gu )[ (long)i * kp + l ] * x[ i ];
+      }
+      __syncthreads();
+   }
+   for ( int e = tid; e < 2 * r * k; e += NT ) {
+      const int i = e % k, c = e / k;
+      const bool isv = c >= r;
+      const int cc = isv ? c - r : c;
+      const double* x = ( isv ? Z : M ) + (long)ord[ cc ] * kp;
+      ( isv ? Cv : Cu )[ (long)cc * kp + i ] = x[ i ] / ( isv ? dv : du )[ i ];
+   }
+   if ( tid == 0 ) rank[ blockIdx.x ] = r, failed[ blockIdx.x ] = 0;
+}
+
+// U' = U Cu (column-major, 3m x r) and V' = V Cv (stored with the rank index fastest) into the
+// block's storage (E).  One workgroup per (job, side, chunk of kTrApply rows), one row per thread,
+// 16 columns of the result per pass with that slice of C staged in LDS.
+template <typename E>
+__global__ void __launch_bounds__( kTrApply ) k_hm_trunc_apply( const int4* __restrict__ tasks, const TruncJob* __restrict__ jobs,
+                                                             const int* __restrict__ rank, void* const* __restrict__ uout,
+                                                             void* const* __restrict__ vout ) {
+   __shared__ double cs[ kTrMaxK * 16 ];
+   const int4 T = tasks[ blockIdx.x ];
+   const TruncJob J = jobs[ T.x ];
+   const int side = T.y, r = rank[ T.x ], k = J.k, kp = J.kp;
+   const int R = side ? J.N : J.M;
+   const double* A = side ? J.V : J.U;
+   const double* C = J.C + ( side ? (long)kp * kp : 0L );
+   E* out = static_cast<E*>( side ? vout[ T.x ] : uout[ T.x ] );
+   const long row = (long)T.z * kTrApply + threadIdx.x;
+   for ( int c0 = 0; c0 < r; c0 += 16 ) {
+      const int nc = min( 16, r - c0 );
+      __syncthreads();
+      for ( int e = threadIdx.x; e < k * 16; e += kTrApply ) {
+         const int l = e / 16, c = e % 16;
+         cs[ e ] = c < nc ? C[ (long)( c0 + c ) * kp + l ] : 0.0;
+      }
+      __syncthreads();
+      if ( row >= R ) continue;
+      double acc[ 16 ];
+#pragma unroll
+      for ( int c = 0; c < 16; ++c ) acc[ c ] = 0.0;
+      for ( int l = 0; l < k; ++l ) {
+         const double u = A[ (long)l * R + row ];
+#pragma unroll
+         for ( int c = 0; c < 16; ++c ) acc[ c ] = fma( u, cs[ l * 16 + c ], acc[ c ] );
+      }
+      if ( side == 0 ) {
+         for ( int c = 0; c < nc; ++c ) out[ (long)( c0 + c ) * R + row ] = (E)acc[ c ];
+      } else {
+         for ( int c = 0; c < nc; ++c ) out[ row * r + c0 + c ] = (E)acc[ c ];
+      }
    }
 }
 
@@ -1058,11 +1414,17 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
       return (double)h.blks[ a ].nr * h.blks[ a ].nc > (double)h.blks[ b ].nr * h.blks[ b ].nc;
    } );
    std::vector<const void*> blk_a( h.blks.size(), nullptr ), blk_b( h.blks.size(), nullptr );
+   long n_aca = 0;
    {
       DevBuf<char> ws;
       DevBuf<AcaJob> jobs_d;
       DevBuf<int> rank_d;
       DevBuf<void*> uo_d, vo_d;
+      DevBuf<double> tr_ws;  // recompression: per-job Gram, rotation and coefficient matrices
+      DevBuf<TruncJob> tj_d;
+      DevBuf<int> trank_d, tfail_d;
+      DevBuf<void*> tuo_d, tvo_d;
+      DevBuf<int4> ttask_d;
       struct Streams {
          hipStream_t s = nullptr;
          hipEvent_t e = nullptr;
@@ -1135,6 +1497,82 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
          std::vector<int> ranks( nj );
          BH_HIP( hipMemcpyAsync( ranks.data(), rank_d.p, sizeof( int ) * nj, hipMemcpyDeviceToHost, st ) );
          BH_HIP( hipStreamSynchronize( st ) );
+         // recompression of the converged approximations of rank 1 .. kTrMaxK
+         const double t_tr = now();
+         std::vector<int> tq;  // batch positions of the recompressed jobs
+         std::vector<TruncJob> tj;
+         std::vector<int> aca_rank = ranks;
+         if ( opt.trunc ) {
+            size_t scratch = 0;
+            // by class (kp <= 32, 64, 96, 128), in batch order (decreasing size) within a class
+            std::vector<int> order;
+            for ( int cls = 0; cls < 4; ++cls )
+               for ( int q = 0; q < nj; ++q ) {
+                  const int k = ranks[ q ];
+                  if ( k >= 1 && k <= kTrMaxK && ( ( k + 3 ) / 4 * 4 + 31 ) / 32 - 1 == cls ) order.push_back( q );
+               }
+            int ncls[ 4 ] = { 0, 0, 0, 0 };
+            for ( int q : order ) {
+               const int k = ranks[ q ];
+               const int kp = ( k + 3 ) / 4 * 4;
+               ++ncls[ ( kp + 31 ) / 32 - 1 ];
+               TruncJob t {};
+               t.U = jobs[ q ].U;
+               t.V = jobs[ q ].V;
+               t.M = 3 * jobs[ q ].m;
+               t.N = 3 * jobs[ q ].nc;
+               t.k = k;
+               t.kp = kp;
+               t.G = reinterpret_cast<double*>( scratch );  // offsets until the scratch exists
+               scratch += 5 * (size_t)kp * kp + 2 * (size_t)kp;
+               tq.push_back( q );
+               tj.push_back( t );
+            }
+            if ( !tj.empty() ) {
+               double* base = tr_ws.ensure( scratch );
+               for ( TruncJob& t : tj ) {
+                  const size_t off = reinterpret_cast<size_t>( t.G );
+                  const size_t kk = (size_t)t.kp * t.kp;
+                  t.G = base + off;
+                  t.Z = t.G + 2 * kk;
+                  t.C = t.Z + kk;
+                  t.d = t.C + 2 * kk;
+               }
+               const int nt = (int)tj.size();
+               tj_d.ensure( nt );
+               trank_d.ensure( nt );
+               tfail_d.ensure( nt );
+               BH_HIP( hipMemcpyAsync( tj_d.p, tj.data(), sizeof( TruncJob ) * nt, hipMemcpyHostToDevice, st ) );
+               // the two classes of large ranks on the second stream, beside the small ones
+               const int o1 = ncls[ 0 ], o2 = o1 + ncls[ 1 ], o3 = o2 + ncls[ 2 ];
+               BH_HIP( hipEventRecord( ev, st ) );
+               BH_HIP( hipStreamWaitEvent( st_big, ev, 0 ) );
+               if ( ncls[ 3 ] )
+                  hipLaunchKernelGGL( ( k_hm_trunc<512, 128> ), dim3( ncls[ 3 ] ), dim3( 512 ), 0, st_big, tj_d.p + o3, opt.eps,
+                                      trank_d.p + o3, tfail_d.p + o3 );
+               if ( ncls[ 2 ] )
+                  hipLaunchKernelGGL( ( k_hm_trunc<512, 96> ), dim3( ncls[ 2 ] ), dim3( 512 ), 0, st_big, tj_d.p + o2, opt.eps,
+                                      trank_d.p + o2, tfail_d.p + o2 );
+               if ( ncls[ 1 ] )
+                  hipLaunchKernelGGL( ( k_hm_trunc<256, 64> ), dim3( ncls[ 1 ] ), dim3( 256 ), 0, st, tj_d.p + o1, opt.eps,
+                                      trank_d.p + o1, tfail_d.p + o1 );
+               if ( ncls[ 0 ] )
+                  hipLaunchKernelGGL( ( k_hm_trunc<128, 32> ), dim3( ncls[ 0 ] ), dim3( 128 ), 0, st, tj_d.p, opt.eps, trank_d.p,
+                                      tfail_d.p );
+               BH_HIP( hipEventRecord( ev, st_big ) );
+               BH_HIP( hipStreamWaitEvent( st, ev, 0 ) );
+               std::vector<int> tr( nt ), tf( nt );
+               BH_HIP( hipMemcpyAsync( tr.data(), trank_d.p, sizeof( int ) * nt, hipMemcpyDeviceToHost, st ) );
+               BH_HIP( hipMemcpyAsync( tf.data(), tfail_d.p, sizeof( int ) * nt, hipMemcpyDeviceToHost, st ) );
+               BH_HIP( hipStreamSynchronize( st ) );
+               for ( int i = 0; i < nt; ++i ) {
+                  ranks[ tq[ i ] ] = tr[ i ];
+                  S.n_trunc_failed += tf[ i ];
+               }
+            }
+         }
+         for ( int q = 0; q < nj; ++q )
+            if ( aca_rank[ q ] > 0 ) S.kmean_aca += aca_rank[ q ], ++n_aca;
          // storage for the converged blocks
          size_t total = 0;
          for ( int q = 0; q < nj; ++q )
@@ -1160,15 +1598,55 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
          }
          uo_d.ensure( nj );
          vo_d.ensure( nj );
+         // the recompressed blocks are written by k_hm_trunc_apply; the rest are copied
+         std::vector<int> crank( nj );
+         for ( int q = 0; q < nj; ++q ) crank[ q ] = ranks[ q ];
+         for ( int q : tq ) crank[ q ] = 0;
          BH_HIP( hipMemcpyAsync( uo_d.p, uo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, st ) );
          BH_HIP( hipMemcpyAsync( vo_d.p, vo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, st ) );
+         BH_HIP( hipMemcpyAsync( rank_d.p, crank.data(), sizeof( int ) * nj, hipMemcpyHostToDevice, st ) );
          if ( opt.single )
             hipLaunchKernelGGL( k_hm_compact<float>, dim3( nj ), dim3( kWg ), 0, st, jobs_d.p, rank_d.p, uo_d.p, vo_d.p );
          else
             hipLaunchKernelGGL( k_hm_compact<double>, dim3( nj ), dim3( kWg ), 0, st, jobs_d.p, rank_d.p, uo_d.p, vo_d.p );
+         if ( !tj.empty() ) {
+            const int nt = (int)tj.size();
+            std::vector<void*> tuo( nt ), tvo( nt );
+            std::vector<int4> tasks;
+            for ( int i = 0; i < nt; ++i ) {
+               tuo[ i ] = uo[ tq[ i ] ];
+               tvo[ i ] = vo[ tq[ i ] ];
+               if ( ranks[ tq[ i ] ] <= 0 ) continue;
+               for ( int side = 0; side < 2; ++side ) {
+                  const int R = side ? tj[ i ].N : tj[ i ].M;
+                  for ( int c = 0; c * kTrApply < R; ++c ) tasks.push_back( make_int4( i, side, c, 0 ) );
+               }
+            }
+            tuo_d.ensure( nt );
+            tvo_d.ensure( nt );
+            BH_HIP( hipMemcpyAsync( tuo_d.p, tuo.data(), sizeof( void* ) * nt, hipMemcpyHostToDevice, st ) );
+            BH_HIP( hipMemcpyAsync( tvo_d.p, tvo.data(), sizeof( void* ) * nt, hipMemcpyHostToDevice, st ) );
+            if ( !tasks.empty() ) {
+               ttask_d.ensure( tasks.size() );
+               BH_HIP( hipMemcpyAsync( ttask_d.p, tasks.data(), sizeof( int4 ) * tasks.size(), hipMemcpyHostToDevice, st ) );
+               if ( opt.single )
+                  hipLaunchKernelGGL( k_hm_trunc_apply<float>, dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, st, ttask_d.p,
+                                      tj_d.p, trank_d.p, tuo_d.p, tvo_d.p );
+               else
+                  hipLaunchKernelGGL( k_hm_trunc_apply<double>, dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, st, ttask_d.p,
+                                      tj_d.p, trank_d.p, tuo_d.p, tvo_d.p );
+            }
+         }
          BH_HIP( hipStreamSynchronize( st ) );
+         S.t_trunc += now() - t_tr;
       }
    }
+
+   S.kmean_aca = n_aca ? S.kmean_aca / n_aca : 0.0;
+   if ( opt.trunc )
+      say( "recompression to eps by QR and SVD: %.3f s, mean rank %.1f after the cross approximation, %ld blocks kept as "
+           "approximated (dependent columns or rank above %d)",
+           S.t_trunc, S.kmean_aca, S.n_trunc_failed, kTrMaxK );
 
    // dense blocks (inadmissible leaves, small admissible blocks, capped approximations)
    {
@@ -1343,6 +1821,20 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
    h.ypart.ensure( (size_t)h.nleaves * kSplit * kRowsWg * kMaxRhs );
    if ( stats ) *stats = S;
    return H;
+}
+
+long hm_blocks( const HMatrix& h, long* out, long cap, int* perm ) {
+   const long nb = (long)h.blks.size();
+   if ( out )
+      for ( long i = 0; i < std::min( nb, cap ); ++i ) {
+         const Blk& b = h.blks[ i ];
+         long* o = out + 6 * i;
+         o[ 0 ] = b.type == kLowRank ? 1 : 0;
+         o[ 1 ] = b.type == kLowRank ? b.rank : 0;
+         o[ 2 ] = b.r0, o[ 3 ] = b.nr, o[ 4 ] = b.c0, o[ 5 ] = b.nc;
+      }
+   if ( perm ) std::copy( h.T.perm.begin(), h.T.perm.end(), perm );
+   return nb;
 }
 
 void hm_apply( HMatrix& h, hipStream_t st, int p, const double* d_x, long ldx, double* d_y, long ldy ) {

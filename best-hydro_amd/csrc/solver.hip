@@ -1064,6 +1064,7 @@ int bhg_set_hm_option( bhg_solver* s, const char* name, double value ) {
    else if ( k == "restart" ) o.restart = (int)value;
    else if ( k == "maxsteps" ) o.maxsteps = (int)value;
    else if ( k == "ws_gb" ) o.ws_bytes = value * 1e9;
+   else if ( k == "trunc" ) o.trunc = value != 0.0 ? 1 : 0;
    else return 1;
    return 0;
 }
@@ -1092,6 +1093,9 @@ int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out ) {
    out->steps = h.steps;
    out->relres = h.relres;
    out->t_solve = h.t_solve;
+   out->n_trunc_failed = h.n_trunc_failed;
+   out->kmean_aca = h.kmean_aca;
+   out->t_trunc = h.t_trunc;
    return 0;
 }
 
@@ -1140,6 +1144,13 @@ int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y,
       BH_HIP( hipMemcpy2DAsync( y, ldy * sizeof( double ), s->x.p, np * sizeof( double ), s->n3 * sizeof( double ), nrhs,
                                 hipMemcpyDeviceToHost, s->st ) );
       BH_HIP( hipStreamSynchronize( s->st ) );
+   } );
+}
+
+int bhg_hm_blocks( bhg_solver* s, long* out, long cap, int* perm, long* count ) {
+   return guarded( s, "bhg_hm_blocks", [&] {
+      if ( !s->hm ) throw std::runtime_error( "no H-matrix" );
+      *count = bhg::hm_blocks( *s->hm, out, cap, perm );
    } );
 }
 

@@ -165,6 +165,8 @@ typedef struct bhg_hm_stats {
    double t_tree, t_fill, t_pre; /* seconds */
    int iters_min, iters_max, steps; /* GMRES iterations per right-hand side, batched products */
    double relres, t_solve;
+   long n_trunc_failed; /* approximations not recompressed (dependent columns or rank above 128) */
+   double kmean_aca, t_trunc; /* mean rank before the recompression; its time (s, part of t_fill) */
 } bhg_hm_stats;
 
 /* method 0: dense LU (default); 1: H-matrix + GMRES (hmatrix/src/api.cpp:585-735) for the next
@@ -179,8 +181,10 @@ int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out );
  * "eta" (admissibility, 2), "aca_factor" (0.1), "leaf" (triangles per leaf cluster, 32),
  * "dense_small" (admissible blocks of at most this many triangle pairs stay dense, 1024), "kcap"
  * (rank at which an approximation is abandoned for a dense block, 192), "restart" (30),
- * "maxsteps" (500), "ws_gb" (device work space of the cross approximations, GB).  Returns 1 for
- * an unknown name. */
+ * "maxsteps" (500), "ws_gb" (device work space of the cross approximations, GB), "trunc" (1:
+ * recompress each approximation to eps by QR and SVD, hmatrix/src/hmatrix.cpp:270-273 and
+ * hmatrix/src/mat.cpp:129-160; 0: keep the approximation at aca_factor * eps).  Returns 1 for an
+ * unknown name. */
 int bhg_set_hm_option( bhg_solver* s, const char* name, double value );
 
 /* The cluster and block trees alone, on the host (hmatrix/src/cluster.cpp:91-112,
@@ -194,6 +198,13 @@ int bhg_hm_plan( int n, const double* v1, const double* v2, const double* v3, co
  * *bytes the algorithmic HBM bytes of one product (the blocks' storage plus X and Y).  No
  * counterpart in the reference: it measures the kernel hmatrix/src/hmatrix.cpp:441-471 runs. */
 int bhg_hm_time_product( bhg_solver* s, int nrhs, int reps, double* ms, double* bytes );
+
+/* The leaf blocks of the built H-matrix, 6 longs each: 1 if low-rank (else 0, dense), rank, first
+ * row, rows, first column, columns (unknowns in tree order); at most cap blocks into out (may be
+ * null), *count = their number.  perm (may be null; n ints) = BEST's triangle (from 0) at each
+ * tree position.  A diagnostic with no counterpart in the reference's C ABI (its HStats,
+ * hmatrix/src/hmatrix.h:58-70, are the aggregates of these). */
+int bhg_hm_blocks( bhg_solver* s, long* out, long cap, int* perm, long* count );
 
 /* y = (GR + deflation) x with the built H-matrix (hmatrix/src/hmatrix.cpp:441-471), BEST's order. */
 int bhg_hm_apply( bhg_solver* s, int nrhs, const double* x, long ldx, double* y, long ldy );

@@ -55,6 +55,57 @@ def test_product_matches_dense(bh, oracle, solver):
     solver.set_method("dense")
 
 
+def test_recompression_ranks_match_svd(bh, oracle, solver):
+    """The recompression of the cross approximations (the reference's truncate: QR of both
+    factors, SVD of Ru Rv^T, singular values above eps * s_max kept; hmatrix/src/mat.cpp:129-160):
+    every low-rank block's rank against the numerical rank at eps of the exact block (numpy's SVD
+    of the oracle's GR entries), the ranks never above the cross approximation's, and the product
+    within 10 eps of the dense one with and without the recompression."""
+    from best_hydro_amd import meshes as M
+    path = os.path.join(GOLDEN, "2eu7_02562")
+    m, _ = _prepared(oracle, "2eu7_02562")
+    n = m["nar"]
+    x_o, gr = oracle.tgetgr(m)
+    grd = gr.copy(order="F")
+    oracle.deflate(m, grd, np.ones(n, dtype=np.int32), 1)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((3 * n, 3))
+    want = grd @ x
+    solver.set_mesh(M.read_best(path))
+    solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+    res = {}
+    for trunc in (0, 1):
+        solver.set_hm_option("trunc", trunc)
+        solver.solve_mesh(precision=64)
+        st = solver.hm_stats()
+        blocks, perm = solver.hm_blocks()
+        err = np.linalg.norm(solver.hm_apply(x) - want) / np.linalg.norm(want)
+        res[trunc] = (st, blocks, perm, err)
+        print(f"trunc={trunc}: {st['n_lowrank']} low-rank blocks, rank max {st['kmax']} mean {st['kmean']:.2f} "
+              f"(cross approximation {st['kmean_aca']:.2f}), {st['bytes_lowrank'] / 1e6:.1f} MB low-rank, "
+              f"{st['n_trunc_failed']} kept, {st['t_trunc']:.3f} s; product error {err:.2e}", flush=True)
+        assert err < 10 * EPS
+    (st0, b0, p0, _), (st1, b1, p1, _) = res[0], res[1]
+    assert np.array_equal(p0, p1) and np.array_equal(b0[:, [0, 2, 3, 4, 5]], b1[:, [0, 2, 3, 4, 5]])
+    lr = b1[:, 0] == 1
+    assert np.all(b1[lr, 1] <= b0[lr, 1])
+    assert st1["bytes_lowrank"] < 0.9 * st0["bytes_lowrank"]
+    assert st1["n_trunc_failed"] <= 0.01 * st1["n_lowrank"]
+    # unknown 3 p + a in tree order is BEST's 3 perm[p] + a
+    idx = (3 * p1.astype(np.int64)[:, None] + np.arange(3)).ravel()
+    diffs = []
+    for t, k, r0, nr, c0, nc in b1[lr]:
+        a = gr[np.ix_(idx[r0:r0 + nr], idx[c0:c0 + nc])]
+        s = np.linalg.svd(a, compute_uv=False)
+        diffs.append(int(k) - int(np.sum(s > EPS * s[0])))
+    diffs = np.array(diffs)
+    print(f"rank - numerical rank at eps over {len(diffs)} blocks: min {diffs.min()} max {diffs.max()}, "
+          f"{np.mean(diffs == 0):.3f} equal, {np.mean(np.abs(diffs) <= 1):.3f} within one", flush=True)
+    assert np.all(np.abs(diffs) <= 2)
+    assert np.mean(np.abs(diffs) <= 1) >= 0.95
+    solver.set_method("dense")
+
+
 @pytest.mark.parametrize("single", [False, True])
 def test_transport_matches_dense_oracle(bh, oracle, solver, single):
     from best_hydro_amd import meshes as M
