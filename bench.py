@@ -157,6 +157,53 @@ def adjoint_c4(bh, M, steps):
     return out
 
 
+def counter_pass(s, M, steps):
+    """The command the HBM traffic figures come from (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE):
+    mixed solves of the workload, then three device-to-device copies of 1 GiB, whose counter values
+    give the scale of each counter (bytes moved / bytes counted)."""
+    import ctypes as C
+    for k in range(steps):
+        r = s.solve_mesh(iarea=1, cg=1.0, ivisco=1, icov=0, precision=32)
+        print(f"counter pass: solve {k}: D_t {r['dtrans']:.10e}", flush=True)
+    s.close()
+    # the HIP runtime the library already loaded (its device-to-device copy kernel)
+    with open("/proc/self/maps") as f:
+        path = next(line.split()[-1] for line in f if "libamdhip64" in line)
+    hip = C.CDLL(path)
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    for rc in (hip.hipMalloc(C.byref(a), nbytes), hip.hipMalloc(C.byref(b), nbytes), hip.hipMemset(a, 0, nbytes)):
+        if rc != 0:
+            raise RuntimeError(f"HIP error {rc} in the calibration copy")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("counter pass: three 1 GiB copies done", flush=True)
+
+
+def traffic_from_profile(kernel_class):
+    """Calibrated HBM bytes per launch of the kernel class, from the committed counter summary
+    (profiles/*_traffic.md, a JSON block written from two rocprofv3 --pmc runs), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
+        with open(path) as f:
+            m = re.search(r"```json\n(.*?)\n```", f.read(), re.S)
+        if not m:
+            continue
+        rec = json.loads(m.group(1))
+        if rec.get("kernel_class") == kernel_class:
+            return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"], os.path.basename(path)
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +213,9 @@ def main():
     ap.add_argument("--no-hmatrix", action="store_true", help="skip the H-matrix path's 81,920-triangle case")
     ap.add_argument("--no-series", action="store_true", help="skip the 8-mesh series")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient")
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="for rocprofv3 --pmc runs: --steps mixed solves of the workload, then three 1 GiB device "
+                         "copies (the calibration of FETCH_SIZE / WRITE_SIZE); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve does not shard across "
@@ -182,6 +232,9 @@ def main():
     n = tri.nar
     s = bh.Solver(0)
     s.set_mesh(tri)  # inputs resident in HBM before any timing
+    if args.counter_pass:
+        counter_pass(s, M, args.steps)
+        return
 
     def run(prec, steps, profile=False):
         s.set_profiling(profile)
@@ -217,9 +270,11 @@ def main():
     flops_per_launch = p["flops"] / p["launches"]
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     peak = PEAKS[flop_classes[dom][0]]
+    traffic, traffic_src = traffic_from_profile(dom)
     roofline = {"bound": "mfma" if dom.startswith("gemm") else "valu", "kernel_class": dom,
                 "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": p["launches"] / args.steps,
+                "traffic": traffic, "traffic_source": traffic_src,
+                "avg_launch_ms": round(avg_ms, 5), "launches_per_step": p["launches"] / args.steps,
                 "flops_per_launch": flops_per_launch,
                 "share_of_step": round(p["ms"] / args.steps / ms32, 3),
                 "profiled_ms_per_step": round(1e3 * elp / args.steps, 3),

@@ -41,6 +41,15 @@ enum SrcField {
    kS21 = 25,  // sqrt(d2121)
 };
 
+// The streams and events of the blocked factorization: st (the caller's; the critical path),
+// st2 (the trailing update), st3 (the U row panels); ev_u / ev_g order st2 after st and back, ev
+// the forks and joins of st3.
+struct LuStreams {
+   hipStream_t st, st2, st3;
+   hipEvent_t ev_u, ev_g;
+   hipEvent_t ev[ 5 ];
+};
+
 // Kernel classes timed by the optional launch profiler.
 enum KernelClass { kKPairs = 0, kKGemm64, kKGemm32, kKDiag, kKTrsm, kKResid, kKOther, kKClasses };
 

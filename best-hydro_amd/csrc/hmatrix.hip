@@ -906,10 +906,10 @@ constexpr int kDotChunkDefl = 8192;  // rows per partial sum of the deflation pr
 constexpr int kBatch = 8;   // loads in flight per lane in the product kernels
 
 // V_b^T X for every low-rank block.  V is stored with the rank index fastest (V[c * rank + l]),
-// so a wave whose lanes are 64 consecutive l reads one contiguous line per column c, and the
-// entries of X it needs, staged in LDS, are the same for every lane.  One wave per task (block,
-// l0, segment of kSeg columns); each lane keeps its P sums in registers and writes them for
-// k_hm_wsum.
+// so a wave whose lanes are consecutive l (and, for ranks below 64, consecutive columns c) reads
+// one contiguous run per step, and the entries of X it needs are staged in LDS.  One wave per
+// task (block, l0, segment of kSeg columns); each lane keeps its P sums in registers and writes
+// them for k_hm_wsum.
 template <typename E, int P>
 __global__ void __launch_bounds__( 64 ) k_hm_vtx( const int4* __restrict__ tasks, const BlkDev* __restrict__ blks,
                                                   const double* __restrict__ X, long ldx, double* __restrict__ part ) {
@@ -917,34 +917,53 @@ __global__ void __launch_bounds__( 64 ) k_hm_vtx( const int4* __restrict__ tasks
    const int t = blockIdx.x, lane = threadIdx.x;
    const int4 T = tasks[ t ];
    const BlkDev B = blks[ T.x ];
-   const int l = T.y + lane;
-   const bool act = l < B.rank;
-   const E* v = static_cast<const E*>( B.b ) + l;
+   // rl rank indices from T.y on; a rank below 64 leaves lanes over: then lane = (column offset
+   // cg, rank index lg) with rpad = rl rounded up to a power of two, G = 64 / rpad consecutive
+   // columns per step (V is stored column after column, the rank index fastest, so the G columns'
+   // entries are one contiguous run)
+   const int rl = min( 64, B.rank - T.y );
+   int rpad = 1;
+   while ( rpad < rl ) rpad <<= 1;
+   const int G = 64 / rpad, lg = lane % rpad, cg = lane / rpad;
+   const int nj = min( kBatch, 256 / G );  // columns per lane per step
+   const bool act = lg < rl;
+   const E* v = static_cast<const E*>( B.b ) + T.y + lg;
    const double* x = X + B.c0;
    const int lo = T.z * kSeg, hi = min( B.nc, lo + kSeg );
    double acc[ P ];
 #pragma unroll
    for ( int q = 0; q < P; ++q ) acc[ q ] = 0.0;
    for ( int cb = lo; cb < hi; cb += 256 ) {
-      const int cn = min( 256, hi - cb ), cn8 = ( cn + kBatch - 1 ) / kBatch * kBatch;
+      const int cn = min( 256, hi - cb ), step = G * nj, cnr = ( cn + step - 1 ) / step * step;
       __syncthreads();
-      for ( int i = lane; i < cn8 * P; i += 64 ) {
-         const int c = i % cn8, q = i / cn8;
+      for ( int i = lane; i < cnr * P; i += 64 ) {
+         const int c = i % cnr, q = i / cnr;
          xs[ c * P + q ] = c < cn ? x[ q * ldx + cb + c ] : 0.0;
       }
       __syncthreads();
-      for ( int c0 = 0; c0 < cn8; c0 += kBatch ) {
+      for ( int c0 = 0; c0 < cnr; c0 += step ) {
          double vv[ kBatch ];
 #pragma unroll
-         for ( int j = 0; j < kBatch; ++j ) vv[ j ] = ( act && c0 + j < cn ) ? (double)v[ (long)( cb + c0 + j ) * B.rank ] : 0.0;
+         for ( int j = 0; j < kBatch; ++j ) {
+            const int c = c0 + j * G + cg;
+            vv[ j ] = ( act && j < nj && c < cn ) ? (double)v[ (long)( cb + c ) * B.rank ] : 0.0;
+         }
 #pragma unroll
-         for ( int j = 0; j < kBatch; ++j )
+         for ( int j = 0; j < kBatch; ++j ) {
+            if ( j >= nj ) break;
+            const int c = c0 + j * G + cg;
 #pragma unroll
-            for ( int q = 0; q < P; ++q ) acc[ q ] = fma( vv[ j ], xs[ ( c0 + j ) * P + q ], acc[ q ] );
+            for ( int q = 0; q < P; ++q ) acc[ q ] = fma( vv[ j ], xs[ c * P + q ], acc[ q ] );
+         }
       }
    }
+   // the G column groups of each rank index, added in a fixed order
+   for ( int o = rpad; o < 64; o <<= 1 )
 #pragma unroll
-   for ( int q = 0; q < P; ++q ) part[ ( (long)t * 64 + lane ) * kMaxRhs + q ] = acc[ q ];
+      for ( int q = 0; q < P; ++q ) acc[ q ] += __shfl_xor( acc[ q ], o );
+   if ( cg == 0 )
+#pragma unroll
+      for ( int q = 0; q < P; ++q ) part[ ( (long)t * 64 + lg ) * kMaxRhs + q ] = acc[ q ];
 }
 
 // W_b(l, q) = sum over the segments, in order, of the partial sums; one thread per (block, l).

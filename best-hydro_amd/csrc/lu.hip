@@ -880,14 +880,16 @@ template void launch_gemm<float>( hipStream_t, int, int, int, float, const float
 // column by the first), then the two U row panels are formed and the trailing matrix gets one
 // rank-256 update, which halves the passes over the trailing matrix of a rank-128 schedule.  The
 // trailing update is split: the next pair of block columns on st, so the next pair's diagonal
-// blocks start at once, and the rest on st2, overlapping them.  ev_u and ev_g order the streams
-// (the U panels of the pair before the update on st2; that update before the next pair's U
-// panels, which read and write rows it updates).  check is set when partial pivoting would have
-// exchanged rows somewhere; the caller must then rerun lu_pivoted on the original matrix.
+// blocks start at once, and the rest on st2, overlapping them.  The U row panels are off the
+// critical path on st3: U(k0, k1) beside the panel below k0, the row panels right of the pair
+// (which wait for the previous pair's update on st2, whose rows they read and write) beside the
+// second block column's factorization, so st runs only diagonal block, panel, update, diagonal
+// block, panel, next-columns update.  check is set when partial pivoting would have exchanged
+// rows somewhere; the caller must then rerun lu_pivoted on the original matrix.
 template <typename T>
-void lu_nopiv( hipStream_t st, hipStream_t st2, hipEvent_t ev_u, hipEvent_t ev_g, int npad, T* A, long lda, T* inv,
-               int* check ) {
+void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check ) {
    const int nb = kNB, nblk = npad / nb;
+   hipStream_t st = S.st, st2 = S.st2, st3 = S.st3;
    auto blk = [&]( int r, int c ) { return A + (long)c * nb * lda + (long)r * nb; };
    auto inv_l = [&]( int k ) { return inv + (long)k * 2 * nb * nb; };
    auto inv_u = [&]( int k ) { return inv + (long)k * 2 * nb * nb + nb * nb; };
@@ -899,48 +901,65 @@ void lu_nopiv( hipStream_t st, hipStream_t st2, hipEvent_t ev_u, hipEvent_t ev_g
       launch_gemm<T>( st, npad - ( k + 1 ) * nb, nb, nb, T( 1 ), blk( k + 1, k ), lda, inv_u( k ), nb, T( 0 ),
                       blk( k + 1, k ), lda, check );
    };
-   auto urow = [&]( int k, int c0, int ncols ) {  // U row panel: L^-1 A over block columns c0..
-      launch_gemm<T>( st, nb, ncols, nb, T( 1 ), inv_l( k ), nb, blk( k, c0 ), lda, T( 0 ), blk( k, c0 ), lda, nullptr );
+   auto urow = [&]( hipStream_t s, int k, int c0, int ncols ) {  // U row panel: L^-1 A over block columns c0..
+      launch_gemm<T>( s, nb, ncols, nb, T( 1 ), inv_l( k ), nb, blk( k, c0 ), lda, T( 0 ), blk( k, c0 ), lda, nullptr );
    };
-   bool pending = false;  // an update on st2 that the next pair's U panels must wait for
+   auto order = [&]( hipEvent_t e, hipStream_t from, hipStream_t to ) {  // to waits for what from has so far
+      BH_HIP( hipEventRecord( e, from ) );
+      BH_HIP( hipStreamWaitEvent( to, e, 0 ) );
+   };
+   bool pending = false;  // an update on st2 that the next pair's U row panels must wait for
    for ( int k0 = 0; k0 < nblk; k0 += 2 ) {
       const int k1 = k0 + 1;
       if ( k1 >= nblk ) {  // a last single block (npad an odd multiple of 128)
          diag( k0 );
          break;
       }
+      const bool last = k1 + 1 == nblk;
+      const int c2 = k1 + 1, rest = npad - c2 * nb;
       diag( k0 );
+      order( S.ev[ 0 ], st, st3 );
+      urow( st3, k0, k1, nb );
+      BH_HIP( hipEventRecord( S.ev[ 1 ], st3 ) );
+      if ( !last ) {
+         if ( pending ) BH_HIP( hipStreamWaitEvent( st3, S.ev_g, 0 ) );
+         pending = false;
+         urow( st3, k0, c2, rest );
+      }
       lpanel( k0 );
-      urow( k0, k1, nb );
+      if ( !last ) {
+         // rows of block k1 right of the pair: A -= L(k1, k0) U(k0, c2..), on st3 once the panel is done
+         order( S.ev[ 2 ], st, st3 );
+         launch_gemm<T>( st3, nb, rest, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( k1, c2 ), lda,
+                         nullptr );
+      }
+      BH_HIP( hipStreamWaitEvent( st, S.ev[ 1 ], 0 ) );
       launch_gemm<T>( st, npad - k1 * nb, nb, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, k1 ), lda, T( 1 ), blk( k1, k1 ), lda,
                       nullptr );
       diag( k1 );
-      if ( k1 + 1 == nblk ) break;
+      if ( last ) break;
+      order( S.ev[ 3 ], st, st3 );
+      urow( st3, k1, c2, rest );
+      BH_HIP( hipEventRecord( S.ev[ 4 ], st3 ) );
       lpanel( k1 );
-      const int c2 = k1 + 1, rest = npad - c2 * nb;
-      if ( pending ) BH_HIP( hipStreamWaitEvent( st, ev_g, 0 ) );
-      pending = false;
-      urow( k0, c2, rest );
-      launch_gemm<T>( st, nb, rest, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( k1, c2 ), lda, nullptr );
-      urow( k1, c2, rest );
+      BH_HIP( hipStreamWaitEvent( st, S.ev[ 4 ], 0 ) );
       // the rank-256 trailing update: L = the two panels below, U = the two row panels
       const int nnext = std::min( 2 * nb, rest );
       launch_gemm<T>( st, rest, nnext, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( c2, c2 ), lda,
                       nullptr );
       if ( rest > nnext ) {
-         BH_HIP( hipEventRecord( ev_u, st ) );
-         BH_HIP( hipStreamWaitEvent( st2, ev_u, 0 ) );
+         order( S.ev_u, st, st2 );
          launch_gemm<T>( st2, rest, rest - nnext, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, c2 + 2 ), lda, T( 1 ),
                          blk( c2, c2 + 2 ), lda, nullptr );
-         BH_HIP( hipEventRecord( ev_g, st2 ) );
+         BH_HIP( hipEventRecord( S.ev_g, st2 ) );
          pending = true;
       }
    }
-   // the caller's later work on st must see st2's last update
-   if ( pending ) BH_HIP( hipStreamWaitEvent( st, ev_g, 0 ) );
+   // the caller's later work on st must see st2's last update (st3's work is joined into st above)
+   if ( pending ) BH_HIP( hipStreamWaitEvent( st, S.ev_g, 0 ) );
 }
-template void lu_nopiv<double>( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, double*, long, double*, int* );
-template void lu_nopiv<float>( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, float*, long, float*, int* );
+template void lu_nopiv<double>( const LuStreams&, int, double*, long, double*, int* );
+template void lu_nopiv<float>( const LuStreams&, int, float*, long, float*, int* );
 
 // True partial pivoting (slow path): the panel is factored column by column with a grid-wide
 // pivot search; interchanges are applied across the whole row at once.

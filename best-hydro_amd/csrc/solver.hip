@@ -38,7 +38,7 @@ void launch_pad_identity( hipStream_t, int, int, double*, long );
 void launch_entries( hipStream_t, int, double, const double*, const double*, const double*, int, int, int, int, double*,
                      long );
 template <typename T>
-void lu_nopiv( hipStream_t, hipStream_t, hipEvent_t, hipEvent_t, int, T*, long, T*, int* );
+void lu_nopiv( const LuStreams&, int, T*, long, T*, int* );
 template <typename T>
 void lu_pivoted( hipStream_t, int, T*, long, T*, int*, int*, double*, int* );
 template <typename T>
@@ -97,7 +97,12 @@ struct bhg_solver {
    int device = 0;
    hipStream_t st = nullptr;
    hipStream_t st2 = nullptr;          // look-ahead stream of the factorization
+   hipStream_t st3 = nullptr;          // its U row panels
    hipEvent_t ev_u = nullptr, ev_g = nullptr;
+   hipEvent_t ev_lu[ 5 ] = {};
+   bhg::LuStreams lu_streams() const {
+      return { st, st2, st3, ev_u, ev_g, { ev_lu[ 0 ], ev_lu[ 1 ], ev_lu[ 2 ], ev_lu[ 3 ], ev_lu[ 4 ] } };
+   }
    std::string last_error;
    std::deque<std::string> log;
    // mesh
@@ -230,7 +235,7 @@ void factor( bhg_solver* s, int precision ) {
    s->transposed = false;
    if ( precision == 64 ) {
       s->inv64.ensure( (size_t)nblk * 2 * kNB * kNB );
-      lu_nopiv<double>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a64.p, (long)np, s->inv64.p, s->flags.p );
+      lu_nopiv<double>( s->lu_streams(), s->npad, s->a64.p, (long)np, s->inv64.p, s->flags.p );
       if ( read_flag( s ) ) {
          s->say( "LU: a multiplier exceeded 1; refactoring with row exchanges" );
          assemble( s, s->deflated );
@@ -247,7 +252,7 @@ void factor( bhg_solver* s, int precision ) {
       s->inv32.ensure( (size_t)nblk * 2 * kNB * kNB );
       convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
       g_growth = kFp32Growth;
-      lu_nopiv<float>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
+      lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
       g_growth = 1.0;
       if ( read_flag( s ) ) {
          s->say( "LU (FP32): a multiplier exceeded 256; refactoring with row exchanges" );
@@ -514,7 +519,7 @@ void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshT
    fill( s->st, s->flags.p, 4, 0 );
    convert_d2f( s->st, np * np, s->a64.p, s->a32.p );
    g_growth = kFp32Growth;
-   lu_nopiv<float>( s->st, s->st2, s->ev_u, s->ev_g, s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
+   lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
    g_growth = 1.0;
    build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
    launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
@@ -912,8 +917,10 @@ bhg_solver* bhg_create( int device ) {
       BH_HIP( hipSetDevice( device ) );
       BH_HIP( hipStreamCreateWithFlags( &s->st, hipStreamNonBlocking ) );
       BH_HIP( hipStreamCreateWithFlags( &s->st2, hipStreamNonBlocking ) );
+      BH_HIP( hipStreamCreateWithFlags( &s->st3, hipStreamNonBlocking ) );
       BH_HIP( hipEventCreateWithFlags( &s->ev_u, hipEventDisableTiming ) );
       BH_HIP( hipEventCreateWithFlags( &s->ev_g, hipEventDisableTiming ) );
+      for ( auto& e : s->ev_lu ) BH_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
       s->timer.init();
       s->flags.ensure( 4 );
    } catch ( ... ) {
@@ -927,13 +934,19 @@ void bhg_destroy( bhg_solver* s ) {
    (void)hipSetDevice( s->device );
    if ( s->st ) (void)hipStreamSynchronize( s->st );
    if ( s->st2 ) (void)hipStreamSynchronize( s->st2 );
-   hipStream_t st = s->st, st2 = s->st2;
+   if ( s->st3 ) (void)hipStreamSynchronize( s->st3 );
+   hipStream_t st = s->st, st2 = s->st2, st3 = s->st3;
    hipEvent_t e1 = s->ev_u, e2 = s->ev_g;
+   hipEvent_t el[ 5 ];
+   std::copy( s->ev_lu, s->ev_lu + 5, el );
    delete s;
    if ( e1 ) (void)hipEventDestroy( e1 );
    if ( e2 ) (void)hipEventDestroy( e2 );
+   for ( hipEvent_t e : el )
+      if ( e ) (void)hipEventDestroy( e );
    if ( st ) (void)hipStreamDestroy( st );
    if ( st2 ) (void)hipStreamDestroy( st2 );
+   if ( st3 ) (void)hipStreamDestroy( st3 );
 }
 
 const char* bhg_last_error( bhg_solver* s ) { return s ? s->last_error.c_str() : "no solver"; }
@@ -1481,7 +1494,7 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
                bhg::fill( m->st, m->flags.p, 4, 0 );
                bhg::convert_d2f( m->st, np * np, m->a64.p, m->a32.p );
                bhg::g_growth = bhg::kFp32Growth;
-               bhg::lu_nopiv<float>( m->st, m->st2, m->ev_u, m->ev_g, m->npad, m->a32.p, np, m->inv32.p, m->flags.p );
+               bhg::lu_nopiv<float>( m->lu_streams(), m->npad, m->a32.p, np, m->inv32.p, m->flags.p );
                bhg::g_growth = 1.0;
                bhg::build_super_inverses<float>( m->st, m->npad, bhg::kSuper, m->a32.p, np, m->inv32.p, m->xl32.p, m->xu32.p,
                                                  m->tmp32.p );
