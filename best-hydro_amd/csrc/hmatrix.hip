@@ -984,6 +984,8 @@ __global__ void k_hm_wsum( const int4* __restrict__ wt, int nwt, const BlkDev* _
 }
 
 constexpr int kSplit = 8;  // waves per leaf cluster in k_hm_rows, each summing a part of its blocks
+constexpr int kRowChunk = 128;  // columns of X (or W) staged at a time in k_hm_rows
+constexpr int kRowBatch = 16;   // loads in flight per lane in k_hm_rows
 
 // Partial Y(rows of leaf L) over part s of the blocks covering those rows (the blocks in a fixed
 // order, cut into kSplit consecutive parts): sums of D x or U w.  One wave per (leaf, part);
@@ -995,7 +997,7 @@ __global__ void __launch_bounds__( 64 ) k_hm_rows( const int* __restrict__ leaf_
                                                    const int* __restrict__ ent_blk, const int* __restrict__ ent_off,
                                                    const BlkDev* __restrict__ blks, const double* __restrict__ X, long ldx,
                                                    const double* __restrict__ W, double* __restrict__ ypart ) {
-   __shared__ double st[ 256 * P ];
+   __shared__ double st[ kRowChunk * P ];
    const int L = blockIdx.x, part = blockIdx.y, r = threadIdx.x;
    const int nr = leaf_nr[ L ];
    const bool act0 = r < nr, act1 = r + 64 < nr;
@@ -1009,8 +1011,8 @@ __global__ void __launch_bounds__( 64 ) k_hm_rows( const int* __restrict__ leaf_
       const E* A = static_cast<const E*>( B.a ) + ent_off[ e ] + r;
       const bool dense = B.type == kDense;
       const int ncol = dense ? B.nc : B.rank;
-      for ( int cb = 0; cb < ncol; cb += 256 ) {
-         const int cn = min( 256, ncol - cb ), cn8 = ( cn + kBatch - 1 ) / kBatch * kBatch;
+      for ( int cb = 0; cb < ncol; cb += kRowChunk ) {
+         const int cn = min( kRowChunk, ncol - cb ), cn8 = ( cn + kRowBatch - 1 ) / kRowBatch * kRowBatch;
          __syncthreads();
          if ( dense ) {
             const double* x = X + B.c0 + cb;
@@ -1023,17 +1025,17 @@ __global__ void __launch_bounds__( 64 ) k_hm_rows( const int* __restrict__ leaf_
             for ( int i = r; i < cn8 * P; i += 64 ) st[ i ] = i < cn * P ? w[ i ] : 0.0;
          }
          __syncthreads();
-         for ( int c0 = 0; c0 < cn8; c0 += kBatch ) {
-            double a0[ kBatch ], a1[ kBatch ];
+         for ( int c0 = 0; c0 < cn8; c0 += kRowBatch ) {
+            double a0[ kRowBatch ], a1[ kRowBatch ];
 #pragma unroll
-            for ( int j = 0; j < kBatch; ++j ) {
+            for ( int j = 0; j < kRowBatch; ++j ) {
                const bool in = c0 + j < cn;
                const long o = (long)( cb + c0 + j ) * B.nr;
                a0[ j ] = ( act0 && in ) ? (double)A[ o ] : 0.0;
                a1[ j ] = ( act1 && in ) ? (double)A[ o + 64 ] : 0.0;
             }
 #pragma unroll
-            for ( int j = 0; j < kBatch; ++j )
+            for ( int j = 0; j < kRowBatch; ++j )
 #pragma unroll
                for ( int q = 0; q < P; ++q ) {
                   const double xv = st[ ( c0 + j ) * P + q ];

@@ -77,7 +77,21 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    }
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave & 1, wn = wave >> 1;
-   const int bm = blockIdx.x, bn = blockIdx.y;
+   // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs (workgroup id % 8), each
+   // XCD with its own L2: give each XCD a contiguous run of the tiles, and order the tiles in
+   // groups of kGroupM block rows, column by column, so that the tiles running together on one
+   // XCD share a few 128-row strips of A and 128-column strips of B in its L2.
+   int bm, bn;
+   {
+      constexpr int kXcd = 8, kGroupM = 8;
+      const int nx = gridDim.x, ntile = nx * gridDim.y;
+      const int id0 = blockIdx.y * nx + blockIdx.x;
+      const int xcd = id0 % kXcd, loc = id0 / kXcd, base = ntile / kXcd, rem = ntile % kXcd;
+      const int id = ( xcd < rem ? xcd * ( base + 1 ) : rem * ( base + 1 ) + ( xcd - rem ) * base ) + loc;
+      const int per = kGroupM * gridDim.y, first = ( id / per ) * kGroupM, gm = min( nx - first, kGroupM );
+      bm = first + ( id % per ) % gm;
+      bn = ( id % per ) / gm;
+   }
    const T* Ab = A + (long)bm * BM;
    const T* Bb = B + (long)bn * BN * ldb;
    constexpr int AE = BM * BK / 256;  // A elements per thread, contiguous along m
