@@ -45,29 +45,30 @@ __device__ inline T tabs( T v ) {
 
 }  // namespace
 
-// C (M x N) = alpha A (M x K) B (K x N) + beta C, column-major; M, N multiples of 128, K of BK.
-// 256 threads: four waves in a 2 x 2 arrangement, each owning a 64 x 64 block of the 128 x 128
-// tile as 4 x 4 MFMA 16 x 16 accumulators.  A and B slices of BK columns pass through LDS in two
-// stages: while one stage feeds the MFMAs, the next slice is already in flight from global memory
-// (one barrier per slice).  In LDS each wave's 64 rows (columns for B) are stored permuted, row
-// 16 t + i at slot 4 i + t, so the four fragments a lane needs for one k come as one 16-byte
-// (FP32) or two 16-byte (FP64) reads.  With beta != 0 the C tile is loaded into registers before
-// the main loop.  C may alias A (when N == 128: each workgroup reads its whole row strip before
-// it writes) or B (when M == 128).  check != nullptr: set *check when any |C| > thr or is not
+// C (M x N) = alpha A (M x K) B (K x N) + beta C, column-major; M, N multiples of the tile (BM x BN,
+// each 64 or 128), K of BK.  256 threads: four waves in a 2 x 2 arrangement, each owning a
+// BM / 2 x BN / 2 block of the tile as MFMA 16 x 16 accumulators (4 x 4 for the 128 x 128 tile).
+// A and B slices of BK columns pass through LDS in two stages: while one stage feeds the MFMAs,
+// the next slice is already in flight from global memory (one barrier per slice).  In LDS each
+// wave's rows (columns for B) are stored permuted, row 16 t + i at slot (W / 16) i + t, so the
+// fragments a lane needs for one k come as one read (16 bytes for FP32 and a 64-row wave block).
+// With beta != 0 the C tile is loaded into registers before the main loop.  C may alias A (when
+// N == BN: each workgroup reads its whole row strip before it writes) or B (when M == BM).  check != nullptr: set *check when any |C| > thr or is not
 // finite (the pivot test of the panel below the diagonal block: thr = 1 is partial pivoting's).
 // Batched over blockIdx.z: batch z = zdiv * q + t offsets A by q sa.hi + t sa.lo, and likewise B, C.
 struct BatchStride {
    long hi = 0, lo = 0;
 };
 
-template <typename T, int BK>
+template <typename T, int BK, int BM = 128, int BN = 128>
 __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
                                                 T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
                                                 BatchStride sb, BatchStride sc ) {
-   constexpr int BM = 128, BN = 128, PAD = 8;
-   constexpr int LDW = BM + PAD;  // LDS row length (elements)
-   __shared__ __attribute__( ( aligned( 16 ) ) ) T As[ 2 ][ BK ][ LDW ];
-   __shared__ __attribute__( ( aligned( 16 ) ) ) T Bs[ 2 ][ BK ][ LDW ];
+   constexpr int PAD = 8;
+   constexpr int WM = BM / 2, WN = BN / 2;  // each wave's block of the tile
+   constexpr int TM = WM / 16, TN = WN / 16;  // its MFMA 16 x 16 blocks
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T As[ 2 ][ BK ][ BM + PAD ];
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T Bs[ 2 ][ BK ][ BN + PAD ];
    using v4 = typename Mfma<T>::v4;
    {
       const long q = blockIdx.z / zdiv, t = blockIdx.z % zdiv;
@@ -80,7 +81,7 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs (workgroup id % 8), each
    // XCD with its own L2: give each XCD a contiguous run of the tiles, and order the tiles in
    // groups of kGroupM block rows, column by column, so that the tiles running together on one
-   // XCD share a few 128-row strips of A and 128-column strips of B in its L2.
+   // XCD share a few row strips of A and column strips of B in its L2.
    int bm, bn;
    {
       constexpr int kXcd = 8, kGroupM = 8;
@@ -100,28 +101,30 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    constexpr int BTPN = BK / BE;      // threads per column of B
    const int a_k = tid / ATPK, a_m = ( tid % ATPK ) * AE;
    const int b_n = tid / BTPN, b_k = ( tid % BTPN ) * BE;
-   // permuted slot of row (column) m of the tile: wave block m / 64, t = (m % 64) / 16, i = m % 16
-   auto slot = []( int m ) { return ( m & ~63 ) + ( ( m & 15 ) << 2 ) + ( ( m >> 4 ) & 3 ); };
-   const int b_slot = slot( b_n );
+   // permuted slot of row (column) m of the tile within its wave block of W rows: t = (m % W) / 16,
+   // i = m % 16 go to slot (W / 16) i + t, so the W / 16 fragments a lane needs for one k are
+   // contiguous (one 16-byte read for FP32 and W = 64)
+   auto slot = []( int m, int W ) { return ( m / W ) * W + ( m & 15 ) * ( W / 16 ) + ( ( m % W ) >> 4 ); };
+   const int b_slot = slot( b_n, WN );
    T ra[ AE ], rb[ BE ];
-   v4 acc[ 4 ][ 4 ];
+   v4 acc[ TM ][ TN ];
 #pragma unroll
-   for ( int t = 0; t < 4; ++t )
+   for ( int t = 0; t < TM; ++t )
 #pragma unroll
-      for ( int u = 0; u < 4; ++u ) acc[ t ][ u ] = v4{ 0, 0, 0, 0 };
+      for ( int u = 0; u < TN; ++u ) acc[ t ][ u ] = v4{ 0, 0, 0, 0 };
    // C tile of this lane, loaded ahead (element r of accumulator (t, u)).
    const int li = lane & 15, lk = lane >> 4;
-   auto crow = [&]( int t, int r ) { return wm * 64 + t * 16 + ( sizeof( T ) == 8 ? lk + 4 * r : 4 * lk + r ); };
+   auto crow = [&]( int t, int r ) { return wm * WM + t * 16 + ( sizeof( T ) == 8 ? lk + 4 * r : 4 * lk + r ); };
    // FP64 keeps its 64 accumulators per lane in AGPRs only without the prefetched tile: there it
    // reads C in the epilogue.
    constexpr bool kPre = sizeof( T ) == 4;
-   T cpre[ 4 ][ 4 ][ 4 ];
+   T cpre[ TM ][ TN ][ 4 ];
    if ( kPre && beta != T( 0 ) ) {
 #pragma unroll
-      for ( int t = 0; t < 4; ++t )
+      for ( int t = 0; t < TM; ++t )
 #pragma unroll
-         for ( int u = 0; u < 4; ++u ) {
-            const T* c = C + (long)( bn * BN + wn * 64 + u * 16 + li ) * ldc + bm * BM;
+         for ( int u = 0; u < TN; ++u ) {
+            const T* c = C + (long)( bn * BN + wn * WN + u * 16 + li ) * ldc + bm * BM;
 #pragma unroll
             for ( int r = 0; r < 4; ++r ) cpre[ t ][ u ][ r ] = c[ crow( t, r ) ];
          }
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    };
    auto sstore = [&]( int stg ) {
 #pragma unroll
-      for ( int e = 0; e < AE; ++e ) As[ stg ][ a_k ][ slot( a_m + e ) ] = ra[ e ];
+      for ( int e = 0; e < AE; ++e ) As[ stg ][ a_k ][ slot( a_m + e, WM ) ] = ra[ e ];
 #pragma unroll
       for ( int e = 0; e < BE; ++e ) Bs[ stg ][ b_k + e ][ b_slot ] = rb[ e ];
    };
@@ -150,25 +153,27 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
 #pragma unroll
       for ( int kk = 0; kk < BK; kk += 4 ) {
          const int kr = kk + lk;
-         const T* pa = &As[ stg ][ kr ][ wm * 64 + li * 4 ];
-         const T* pb = &Bs[ stg ][ kr ][ wn * 64 + li * 4 ];
-         T af[ 4 ], bf[ 4 ];
+         const T* pa = &As[ stg ][ kr ][ wm * WM + li * TM ];
+         const T* pb = &Bs[ stg ][ kr ][ wn * WN + li * TN ];
+         T af[ TM ], bf[ TN ];
 #pragma unroll
-         for ( int t = 0; t < 4; ++t ) af[ t ] = pa[ t ], bf[ t ] = pb[ t ];
+         for ( int t = 0; t < TM; ++t ) af[ t ] = pa[ t ];
 #pragma unroll
-         for ( int t = 0; t < 4; ++t )
+         for ( int u = 0; u < TN; ++u ) bf[ u ] = pb[ u ];
 #pragma unroll
-            for ( int u = 0; u < 4; ++u ) acc[ t ][ u ] = Mfma<T>::run( af[ t ], bf[ u ], acc[ t ][ u ] );
+         for ( int t = 0; t < TM; ++t )
+#pragma unroll
+            for ( int u = 0; u < TN; ++u ) acc[ t ][ u ] = Mfma<T>::run( af[ t ], bf[ u ], acc[ t ][ u ] );
       }
       if ( kt + 1 < nk ) sstore( stg ^ 1 );
       __syncthreads();
    }
    bool big = false;
 #pragma unroll
-   for ( int t = 0; t < 4; ++t )
+   for ( int t = 0; t < TM; ++t )
 #pragma unroll
-      for ( int u = 0; u < 4; ++u ) {
-         T* c = C + (long)( bn * BN + wn * 64 + u * 16 + li ) * ldc + bm * BM;
+      for ( int u = 0; u < TN; ++u ) {
+         T* c = C + (long)( bn * BN + wn * WN + u * 16 + li ) * ldc + bm * BM;
 #pragma unroll
          for ( int r = 0; r < 4; ++r ) {
             T v = alpha * acc[ t ][ u ][ r ];
@@ -857,25 +862,44 @@ thread_local int g_gemm_bk_f32 = 32;
 // factorization that iterative refinement corrects accepts more.  Set per call on this thread.
 thread_local double g_growth = 1.0;
 
+template <typename T, int BM, int BN>
+void launch_gemm_shape( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta, T* C,
+                        long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb, BatchStride sc ) {
+   dim3 grid( M / BM, N / BN, batch );
+   if constexpr ( sizeof( T ) == 8 )
+      hipLaunchKernelGGL( ( k_gemm<double, 16, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                          check, T( g_growth ), zdiv, sa, sb, sc );
+   else if ( g_gemm_bk_f32 == 16 || K % 32 != 0 )
+      hipLaunchKernelGGL( ( k_gemm<float, 16, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                          check, T( g_growth ), zdiv, sa, sb, sc );
+   else if ( g_gemm_bk_f32 == 64 && K % 64 == 0 )
+      hipLaunchKernelGGL( ( k_gemm<float, 64, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                          check, T( g_growth ), zdiv, sa, sb, sc );
+   else
+      hipLaunchKernelGGL( ( k_gemm<float, 32, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                          check, T( g_growth ), zdiv, sa, sb, sc );
+}
+
+// Tiles of 128 x 128 where they fill the chip; the small products of the factorization's critical
+// path (fewer 128 x 128 tiles than CUs) take 64-row and/or 64-column tiles instead, two to four
+// times the workgroups for the same work.  A product written in place keeps its whole aliased
+// dimension in one tile: over A (C == A) all N columns (N == 128), over B (C == B) all M rows.
 template <typename T>
 void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta,
                           T* C, long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb,
                           BatchStride sc ) {
    if ( M <= 0 || N <= 0 || batch <= 0 ) return;
-   dim3 grid( M / 128, N / 128, batch );
    ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32, 2.0 * M * N * K * batch, st );
-   if constexpr ( sizeof( T ) == 8 )
-      hipLaunchKernelGGL( ( k_gemm<double, 16> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
-                          T( g_growth ), zdiv, sa, sb, sc );
-   else if ( g_gemm_bk_f32 == 16 || K % 32 != 0 )
-      hipLaunchKernelGGL( ( k_gemm<float, 16> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
-                          T( g_growth ), zdiv, sa, sb, sc );
-   else if ( g_gemm_bk_f32 == 64 && K % 64 == 0 )
-      hipLaunchKernelGGL( ( k_gemm<float, 64> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
-                          T( g_growth ), zdiv, sa, sb, sc );
+   const long tiles = (long)( M / 128 ) * ( N / 128 ) * batch;
+   constexpr long kSmall = 256;  // CUs
+   if ( tiles < kSmall && C == A && N == 128 )
+      launch_gemm_shape<T, 64, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+   else if ( tiles < kSmall && C == B && M == 128 )
+      launch_gemm_shape<T, 128, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+   else if ( tiles < kSmall && C != A && C != B )
+      launch_gemm_shape<T, 64, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
    else
-      hipLaunchKernelGGL( ( k_gemm<float, 32> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check,
-                          T( g_growth ), zdiv, sa, sb, sc );
+      launch_gemm_shape<T, 128, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
 }
 
 template <typename T>
