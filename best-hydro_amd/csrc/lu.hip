@@ -297,8 +297,13 @@ __device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i, T thr ) {
       const bool below = i > j;
       big |= below && !( tabs( a[ j ] ) <= thr * tabs( piv ) );
       const T l = below ? a[ j ] * rp : T( 0 );
+      // the pivot row first, then the updates: the broadcasts land in distinct scalar registers
+      // and issue back to back instead of alternating with the FMAs that read them
+      T prow[ 32 ];
 #pragma unroll
-      for ( int c = j + 1; c < 32; ++c ) a[ c ] = a[ c ] - l * bcast( a[ c ], j );
+      for ( int c = j + 1; c < 32; ++c ) prow[ c ] = bcast( a[ c ], j );
+#pragma unroll
+      for ( int c = j + 1; c < 32; ++c ) a[ c ] = a[ c ] - l * prow[ c ];
       a[ j ] = below ? l : a[ j ];
    }
    return big;
@@ -311,8 +316,11 @@ __device__ __forceinline__ void leaf_inv_lower( T ( &a )[ 32 ], int i ) {
 #pragma unroll
    for ( int k = 0; k < 32; ++k ) {
       const T li = i > k ? a[ k ] : T( 0 );
+      T prow[ 32 ];
 #pragma unroll
-      for ( int c = 0; c < k; ++c ) a[ c ] = a[ c ] - li * bcast( a[ c ], k );
+      for ( int c = 0; c < k; ++c ) prow[ c ] = bcast( a[ c ], k );
+#pragma unroll
+      for ( int c = 0; c < k; ++c ) a[ c ] = a[ c ] - li * prow[ c ];
       a[ k ] = i > k ? -li : a[ k ];
    }
 }
@@ -332,14 +340,20 @@ __device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i ) {
 #pragma unroll
    for ( int k = 31; k > 0; --k ) {
       const T ui = i < k ? a[ k ] : T( 0 );
+      T prow[ 32 ];
 #pragma unroll
-      for ( int c = k + 1; c < 32; ++c ) a[ c ] = a[ c ] - ui * bcast( a[ c ], k );
+      for ( int c = k + 1; c < 32; ++c ) prow[ c ] = bcast( a[ c ], k );
+#pragma unroll
+      for ( int c = k + 1; c < 32; ++c ) a[ c ] = a[ c ] - ui * prow[ c ];
       a[ k ] = i < k ? -ui : a[ k ];
    }
+   T rdc[ 32 ];
+#pragma unroll
+   for ( int c = 0; c < 32; ++c ) rdc[ c ] = bcast( rd, c );
 #pragma unroll
    for ( int c = 0; c < 32; ++c ) {
       const T x = i == c ? T( 1 ) : a[ c ];
-      a[ c ] = c >= i ? x * bcast( rd, c ) : a[ c ];
+      a[ c ] = c >= i ? x * rdc[ c ] : a[ c ];
    }
 }
 
