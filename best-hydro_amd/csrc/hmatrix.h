@@ -70,8 +70,10 @@ struct HMatrixDeleter {
 };
 using HMatrixPtr = std::unique_ptr<HMatrix, HMatrixDeleter>;
 
-// Builds the H-matrix of GR + sum_k gdef[k] xmu_k xmu_k^T and the GMRES preconditioner.
-HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, HmStats* stats,
+// Builds the H-matrix of GR + sum_k gdef[k] xmu_k xmu_k^T and the GMRES preconditioner, on st
+// with two more streams beside it (the largest blocks' approximations, the recompression of large
+// ranks); the three should sit on different hardware queues.
+HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const HmInput& in, const HmOptions& opt, HmStats* stats,
                      std::vector<std::string>* log );
 
 // Solves for nrhs (<= 16) right-hand sides: b and x are device arrays in BEST's unknown order

@@ -1380,7 +1380,8 @@ HmStats hm_plan( int n, const double* h_v, const double* h_xc, const HmOptions& 
    return S;
 }
 
-HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, HmStats* stats, std::vector<std::string>* log ) {
+HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const HmInput& in, const HmOptions& opt, HmStats* stats,
+                     std::vector<std::string>* log ) {
    auto say = [&]( const char* fmt, auto... a ) {
       char buf[ 512 ];
       std::snprintf( buf, sizeof buf, fmt, a... );
@@ -1427,7 +1428,10 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
    BH_HIP( hipMemsetAsync( calls.p, 0, sizeof( unsigned long long ), st ) );
    const size_t esz = opt.single ? sizeof( float ) : sizeof( double );
 
-   // low-rank blocks: cross approximation in batches, largest first
+   // low-rank blocks: cross approximation in batches, largest first.  The largest blocks (3 (m + n)
+   // >= kAcaBig: a few hundred, each a 512-thread workgroup that runs for up to hundreds of ms)
+   // form a batch of their own on a second stream, launched first and finished last, so that the
+   // other batches and their recompression run beside them instead of waiting for them.
    std::vector<int> lr;
    for ( int i = 0; i < (int)h.blks.size(); ++i )
       if ( h.blks[ i ].type == kLowRank ) lr.push_back( i );
@@ -1437,54 +1441,54 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
    std::vector<const void*> blk_a( h.blks.size(), nullptr ), blk_b( h.blks.size(), nullptr );
    long n_aca = 0;
    {
-      DevBuf<char> ws;
-      DevBuf<AcaJob> jobs_d;
-      DevBuf<int> rank_d;
-      DevBuf<void*> uo_d, vo_d;
-      DevBuf<double> tr_ws;  // recompression: per-job Gram, rotation and coefficient matrices
-      DevBuf<TruncJob> tj_d;
-      DevBuf<int> trank_d, tfail_d;
-      DevBuf<void*> tuo_d, tvo_d;
-      DevBuf<int4> ttask_d;
-      struct Streams {
-         hipStream_t s = nullptr;
+      struct Ev {
          hipEvent_t e = nullptr;
-         ~Streams() {
+         ~Ev() {
             if ( e ) (void)hipEventDestroy( e );
-            if ( s ) (void)hipStreamDestroy( s );
          }
       } aux;
-      BH_HIP( hipStreamCreateWithFlags( &aux.s, hipStreamNonBlocking ) );
       BH_HIP( hipEventCreateWithFlags( &aux.e, hipEventDisableTiming ) );
-      hipStream_t st_big = aux.s;
+      hipStream_t st_big = st_b, st_aux = st_c;
       hipEvent_t ev = aux.e;
-      size_t pos = 0;
-      while ( pos < lr.size() ) {
-         // a batch within the work-space budget
+      struct Batch {
+         std::vector<int> idx;  // positions in lr
          std::vector<AcaJob> jobs;
+         DevBuf<char> ws;
+         DevBuf<AcaJob> jobs_d;
+         DevBuf<int> rank_d;
+         DevBuf<void*> uo_d, vo_d;
+         DevBuf<double> tr_ws;  // recompression: per-job Gram, rotation and coefficient matrices
+         DevBuf<TruncJob> tj_d;
+         DevBuf<int> trank_d, tfail_d;
+         DevBuf<void*> tuo_d, tvo_d;
+         DevBuf<int4> ttask_d;
+      };
+      auto kcap_of = [&]( const Blk& b ) { return std::max( 3, (int)std::min<long>( opt.kcap, std::min( b.nr, b.nc ) / 2 ) ); };
+      auto need_of = [&]( const Blk& b ) {
+         return sizeof( double ) * ( (size_t)( b.nr + b.nc ) * kcap_of( b ) + 3 * b.nc + b.nr + b.nc ) +
+                sizeof( int ) * ( b.nr / 3 + 2 ) + 256;
+      };
+      // Work space and jobs of the batch's blocks, then the approximations: the 512-thread prefix
+      // (the batch is in decreasing size) on s2, the rest on s (s2 == s: all on s).
+      auto launch = [&]( Batch& B, hipStream_t s, hipStream_t s2 ) {
+         B.jobs.clear();
          std::vector<size_t> offs;
          size_t bytes = 0;
-         const size_t first = pos;
-         while ( pos < lr.size() ) {
-            const Blk& b = h.blks[ lr[ pos ] ];
-            const int kcap = std::max( 3, (int)std::min<long>( opt.kcap, std::min( b.nr, b.nc ) / 2 ) );
-            const size_t need = sizeof( double ) * ( (size_t)( b.nr + b.nc ) * kcap + 3 * b.nc + b.nr + b.nc ) +
-                                sizeof( int ) * ( b.nr / 3 + 2 ) + 256;
-            if ( pos > first && bytes + need > opt.ws_bytes ) break;
+         for ( int p : B.idx ) {
+            const Blk& b = h.blks[ lr[ p ] ];
             AcaJob j {};
             j.rb = (int)( b.r0 / 3 );
             j.m = (int)( b.nr / 3 );
             j.cb = (int)( b.c0 / 3 );
             j.nc = (int)( b.nc / 3 );
-            j.kcap = kcap;
+            j.kcap = kcap_of( b );
             offs.push_back( bytes );
-            bytes += ( need + 255 ) / 256 * 256;
-            jobs.push_back( j );
-            ++pos;
+            bytes += ( need_of( b ) + 255 ) / 256 * 256;
+            B.jobs.push_back( j );
          }
-         char* base = ws.ensure( bytes );
-         for ( size_t q = 0; q < jobs.size(); ++q ) {
-            AcaJob& j = jobs[ q ];
+         char* base = B.ws.ensure( bytes );
+         for ( size_t q = 0; q < B.jobs.size(); ++q ) {
+            AcaJob& j = B.jobs[ q ];
             double* d = reinterpret_cast<double*>( base + offs[ q ] );
             const long M = 3L * j.m, N = 3L * j.nc;
             j.U = d;
@@ -1494,31 +1498,35 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
             j.v = j.u + M;
             j.used = reinterpret_cast<int*>( j.v + N );
          }
-         const int nj = (int)jobs.size();
-         jobs_d.ensure( nj );
-         rank_d.ensure( nj );
-         BH_HIP( hipMemcpyAsync( jobs_d.p, jobs.data(), sizeof( AcaJob ) * nj, hipMemcpyHostToDevice, st ) );
-         // the large blocks (a prefix: the batch is in decreasing size) get 512 threads, on a
-         // second stream so the small blocks run beside them
+         const int nj = (int)B.jobs.size();
+         B.jobs_d.ensure( nj );
+         B.rank_d.ensure( nj );
+         BH_HIP( hipMemcpyAsync( B.jobs_d.p, B.jobs.data(), sizeof( AcaJob ) * nj, hipMemcpyHostToDevice, s ) );
          int nbig = 0;
-         while ( nbig < nj && 3L * ( jobs[ nbig ].m + jobs[ nbig ].nc ) >= kAcaBig ) ++nbig;
-         if ( nbig > 0 ) {
-            BH_HIP( hipEventRecord( ev, st ) );
-            BH_HIP( hipStreamWaitEvent( st_big, ev, 0 ) );
-            hipLaunchKernelGGL( k_hm_aca<512>, dim3( nbig ), dim3( 512 ), 0, st_big, jobs_d.p, K, opt.aca_factor * opt.eps,
-                                rank_d.p, calls.p );
+         while ( nbig < nj && 3L * ( B.jobs[ nbig ].m + B.jobs[ nbig ].nc ) >= kAcaBig ) ++nbig;
+         if ( nbig > 0 && s2 != s ) {
+            BH_HIP( hipEventRecord( ev, s ) );
+            BH_HIP( hipStreamWaitEvent( s2, ev, 0 ) );
          }
+         if ( nbig > 0 )
+            hipLaunchKernelGGL( k_hm_aca<512>, dim3( nbig ), dim3( 512 ), 0, s2, B.jobs_d.p, K, opt.aca_factor * opt.eps,
+                                B.rank_d.p, calls.p );
          if ( nj > nbig )
-            hipLaunchKernelGGL( k_hm_aca<kWg>, dim3( nj - nbig ), dim3( kWg ), 0, st, jobs_d.p + nbig, K, opt.aca_factor * opt.eps,
-                                rank_d.p + nbig, calls.p );
-         if ( nbig > 0 ) {
-            BH_HIP( hipEventRecord( ev, st_big ) );
-            BH_HIP( hipStreamWaitEvent( st, ev, 0 ) );
+            hipLaunchKernelGGL( k_hm_aca<kWg>, dim3( nj - nbig ), dim3( kWg ), 0, s, B.jobs_d.p + nbig, K,
+                                opt.aca_factor * opt.eps, B.rank_d.p + nbig, calls.p );
+         if ( nbig > 0 && s2 != s ) {
+            BH_HIP( hipEventRecord( ev, s2 ) );
+            BH_HIP( hipStreamWaitEvent( s, ev, 0 ) );
          }
+      };
+      // Waits for the batch's approximations, recompresses them (the two classes of large ranks on
+      // s2), then stores every block (the recompressed ones by k_hm_trunc_apply, the rest copied).
+      auto finish = [&]( Batch& B, hipStream_t s, hipStream_t s2 ) {
+         const int nj = (int)B.jobs.size();
+         const std::vector<AcaJob>& jobs = B.jobs;
          std::vector<int> ranks( nj );
-         BH_HIP( hipMemcpyAsync( ranks.data(), rank_d.p, sizeof( int ) * nj, hipMemcpyDeviceToHost, st ) );
-         BH_HIP( hipStreamSynchronize( st ) );
-         // recompression of the converged approximations of rank 1 .. kTrMaxK
+         BH_HIP( hipMemcpyAsync( ranks.data(), B.rank_d.p, sizeof( int ) * nj, hipMemcpyDeviceToHost, s ) );
+         BH_HIP( hipStreamSynchronize( s ) );
          const double t_tr = now();
          std::vector<int> tq;  // batch positions of the recompressed jobs
          std::vector<TruncJob> tj;
@@ -1550,7 +1558,7 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
                tj.push_back( t );
             }
             if ( !tj.empty() ) {
-               double* base = tr_ws.ensure( scratch );
+               double* base = B.tr_ws.ensure( scratch );
                for ( TruncJob& t : tj ) {
                   const size_t off = reinterpret_cast<size_t>( t.G );
                   const size_t kk = (size_t)t.kp * t.kp;
@@ -1560,32 +1568,31 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
                   t.d = t.C + 2 * kk;
                }
                const int nt = (int)tj.size();
-               tj_d.ensure( nt );
-               trank_d.ensure( nt );
-               tfail_d.ensure( nt );
-               BH_HIP( hipMemcpyAsync( tj_d.p, tj.data(), sizeof( TruncJob ) * nt, hipMemcpyHostToDevice, st ) );
-               // the two classes of large ranks on the second stream, beside the small ones
+               B.tj_d.ensure( nt );
+               B.trank_d.ensure( nt );
+               B.tfail_d.ensure( nt );
+               BH_HIP( hipMemcpyAsync( B.tj_d.p, tj.data(), sizeof( TruncJob ) * nt, hipMemcpyHostToDevice, s ) );
                const int o1 = ncls[ 0 ], o2 = o1 + ncls[ 1 ], o3 = o2 + ncls[ 2 ];
-               BH_HIP( hipEventRecord( ev, st ) );
-               BH_HIP( hipStreamWaitEvent( st_big, ev, 0 ) );
+               BH_HIP( hipEventRecord( ev, s ) );
+               BH_HIP( hipStreamWaitEvent( s2, ev, 0 ) );
                if ( ncls[ 3 ] )
-                  hipLaunchKernelGGL( ( k_hm_trunc<512, 128> ), dim3( ncls[ 3 ] ), dim3( 512 ), 0, st_big, tj_d.p + o3, opt.eps,
-                                      trank_d.p + o3, tfail_d.p + o3 );
+                  hipLaunchKernelGGL( ( k_hm_trunc<512, 128> ), dim3( ncls[ 3 ] ), dim3( 512 ), 0, s2, B.tj_d.p + o3, opt.eps,
+                                      B.trank_d.p + o3, B.tfail_d.p + o3 );
                if ( ncls[ 2 ] )
-                  hipLaunchKernelGGL( ( k_hm_trunc<512, 96> ), dim3( ncls[ 2 ] ), dim3( 512 ), 0, st_big, tj_d.p + o2, opt.eps,
-                                      trank_d.p + o2, tfail_d.p + o2 );
+                  hipLaunchKernelGGL( ( k_hm_trunc<512, 96> ), dim3( ncls[ 2 ] ), dim3( 512 ), 0, s2, B.tj_d.p + o2, opt.eps,
+                                      B.trank_d.p + o2, B.tfail_d.p + o2 );
                if ( ncls[ 1 ] )
-                  hipLaunchKernelGGL( ( k_hm_trunc<256, 64> ), dim3( ncls[ 1 ] ), dim3( 256 ), 0, st, tj_d.p + o1, opt.eps,
-                                      trank_d.p + o1, tfail_d.p + o1 );
+                  hipLaunchKernelGGL( ( k_hm_trunc<256, 64> ), dim3( ncls[ 1 ] ), dim3( 256 ), 0, s, B.tj_d.p + o1, opt.eps,
+                                      B.trank_d.p + o1, B.tfail_d.p + o1 );
                if ( ncls[ 0 ] )
-                  hipLaunchKernelGGL( ( k_hm_trunc<128, 32> ), dim3( ncls[ 0 ] ), dim3( 128 ), 0, st, tj_d.p, opt.eps, trank_d.p,
-                                      tfail_d.p );
-               BH_HIP( hipEventRecord( ev, st_big ) );
-               BH_HIP( hipStreamWaitEvent( st, ev, 0 ) );
+                  hipLaunchKernelGGL( ( k_hm_trunc<128, 32> ), dim3( ncls[ 0 ] ), dim3( 128 ), 0, s, B.tj_d.p, opt.eps,
+                                      B.trank_d.p, B.tfail_d.p );
+               BH_HIP( hipEventRecord( ev, s2 ) );
+               BH_HIP( hipStreamWaitEvent( s, ev, 0 ) );
                std::vector<int> tr( nt ), tf( nt );
-               BH_HIP( hipMemcpyAsync( tr.data(), trank_d.p, sizeof( int ) * nt, hipMemcpyDeviceToHost, st ) );
-               BH_HIP( hipMemcpyAsync( tf.data(), tfail_d.p, sizeof( int ) * nt, hipMemcpyDeviceToHost, st ) );
-               BH_HIP( hipStreamSynchronize( st ) );
+               BH_HIP( hipMemcpyAsync( tr.data(), B.trank_d.p, sizeof( int ) * nt, hipMemcpyDeviceToHost, s ) );
+               BH_HIP( hipMemcpyAsync( tf.data(), B.tfail_d.p, sizeof( int ) * nt, hipMemcpyDeviceToHost, s ) );
+               BH_HIP( hipStreamSynchronize( s ) );
                for ( int i = 0; i < nt; ++i ) {
                   ranks[ tq[ i ] ] = tr[ i ];
                   S.n_trunc_failed += tf[ i ];
@@ -1602,7 +1609,8 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
          std::vector<void*> uo( nj, nullptr ), vo( nj, nullptr );
          size_t at = 0;
          for ( int q = 0; q < nj; ++q ) {
-            Blk& b = h.blks[ lr[ first + q ] ];
+            const int bi = lr[ B.idx[ q ] ];
+            Blk& b = h.blks[ bi ];
             if ( ranks[ q ] < 0 ) {
                b.type = kDense;  // reached kcap: kept dense
                b.capped = true;
@@ -1614,22 +1622,22 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
             uo[ q ] = store + at;
             vo[ q ] = store + at + (size_t)b.nr * b.rank * esz;
             at += ( (size_t)( b.nr + b.nc ) * b.rank * esz + 255 ) / 256 * 256;
-            blk_a[ lr[ first + q ] ] = uo[ q ];
-            blk_b[ lr[ first + q ] ] = vo[ q ];
+            blk_a[ bi ] = uo[ q ];
+            blk_b[ bi ] = vo[ q ];
          }
-         uo_d.ensure( nj );
-         vo_d.ensure( nj );
+         B.uo_d.ensure( nj );
+         B.vo_d.ensure( nj );
          // the recompressed blocks are written by k_hm_trunc_apply; the rest are copied
          std::vector<int> crank( nj );
          for ( int q = 0; q < nj; ++q ) crank[ q ] = ranks[ q ];
          for ( int q : tq ) crank[ q ] = 0;
-         BH_HIP( hipMemcpyAsync( uo_d.p, uo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, st ) );
-         BH_HIP( hipMemcpyAsync( vo_d.p, vo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, st ) );
-         BH_HIP( hipMemcpyAsync( rank_d.p, crank.data(), sizeof( int ) * nj, hipMemcpyHostToDevice, st ) );
+         BH_HIP( hipMemcpyAsync( B.uo_d.p, uo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, s ) );
+         BH_HIP( hipMemcpyAsync( B.vo_d.p, vo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, s ) );
+         BH_HIP( hipMemcpyAsync( B.rank_d.p, crank.data(), sizeof( int ) * nj, hipMemcpyHostToDevice, s ) );
          if ( opt.single )
-            hipLaunchKernelGGL( k_hm_compact<float>, dim3( nj ), dim3( kWg ), 0, st, jobs_d.p, rank_d.p, uo_d.p, vo_d.p );
+            hipLaunchKernelGGL( k_hm_compact<float>, dim3( nj ), dim3( kWg ), 0, s, B.jobs_d.p, B.rank_d.p, B.uo_d.p, B.vo_d.p );
          else
-            hipLaunchKernelGGL( k_hm_compact<double>, dim3( nj ), dim3( kWg ), 0, st, jobs_d.p, rank_d.p, uo_d.p, vo_d.p );
+            hipLaunchKernelGGL( k_hm_compact<double>, dim3( nj ), dim3( kWg ), 0, s, B.jobs_d.p, B.rank_d.p, B.uo_d.p, B.vo_d.p );
          if ( !tj.empty() ) {
             const int nt = (int)tj.size();
             std::vector<void*> tuo( nt ), tvo( nt );
@@ -1643,23 +1651,86 @@ HMatrixPtr hm_build( hipStream_t st, const HmInput& in, const HmOptions& opt, Hm
                   for ( int c = 0; c * kTrApply < R; ++c ) tasks.push_back( make_int4( i, side, c, 0 ) );
                }
             }
-            tuo_d.ensure( nt );
-            tvo_d.ensure( nt );
-            BH_HIP( hipMemcpyAsync( tuo_d.p, tuo.data(), sizeof( void* ) * nt, hipMemcpyHostToDevice, st ) );
-            BH_HIP( hipMemcpyAsync( tvo_d.p, tvo.data(), sizeof( void* ) * nt, hipMemcpyHostToDevice, st ) );
+            B.tuo_d.ensure( nt );
+            B.tvo_d.ensure( nt );
+            BH_HIP( hipMemcpyAsync( B.tuo_d.p, tuo.data(), sizeof( void* ) * nt, hipMemcpyHostToDevice, s ) );
+            BH_HIP( hipMemcpyAsync( B.tvo_d.p, tvo.data(), sizeof( void* ) * nt, hipMemcpyHostToDevice, s ) );
             if ( !tasks.empty() ) {
-               ttask_d.ensure( tasks.size() );
-               BH_HIP( hipMemcpyAsync( ttask_d.p, tasks.data(), sizeof( int4 ) * tasks.size(), hipMemcpyHostToDevice, st ) );
+               B.ttask_d.ensure( tasks.size() );
+               BH_HIP( hipMemcpyAsync( B.ttask_d.p, tasks.data(), sizeof( int4 ) * tasks.size(), hipMemcpyHostToDevice, s ) );
                if ( opt.single )
-                  hipLaunchKernelGGL( k_hm_trunc_apply<float>, dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, st, ttask_d.p,
-                                      tj_d.p, trank_d.p, tuo_d.p, tvo_d.p );
+                  hipLaunchKernelGGL( k_hm_trunc_apply<float>, dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, s, B.ttask_d.p,
+                                      B.tj_d.p, B.trank_d.p, B.tuo_d.p, B.tvo_d.p );
                else
-                  hipLaunchKernelGGL( k_hm_trunc_apply<double>, dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, st, ttask_d.p,
-                                      tj_d.p, trank_d.p, tuo_d.p, tvo_d.p );
+                  hipLaunchKernelGGL( k_hm_trunc_apply<double>, dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, s, B.ttask_d.p,
+                                      B.tj_d.p, B.trank_d.p, B.tuo_d.p, B.tvo_d.p );
             }
          }
-         BH_HIP( hipStreamSynchronize( st ) );
+         BH_HIP( hipStreamSynchronize( s ) );
          S.t_trunc += now() - t_tr;
+      };
+      // the largest blocks first, on their own stream (within the work-space budget)
+      Batch big;
+      size_t pos = 0, big_bytes = 0;
+      while ( pos < lr.size() ) {
+         const Blk& b = h.blks[ lr[ pos ] ];
+         if ( b.nr + b.nc < kAcaBig || ( !big.idx.empty() && big_bytes + need_of( b ) > opt.ws_bytes ) ) break;
+         big_bytes += ( need_of( b ) + 255 ) / 256 * 256;
+         big.idx.push_back( (int)pos++ );
+      }
+      if ( !big.idx.empty() ) {
+         BH_HIP( hipEventRecord( ev, st ) );  // the tree-ordered data and the counter are ready
+         BH_HIP( hipStreamWaitEvent( st_big, ev, 0 ) );
+         launch( big, st_big, st_big );
+      }
+      // the rest in batches within the budget, each finished before the next.  The batches are
+      // cut first and the shared buffers sized for the largest, so that none of them is freed
+      // and reallocated while the large blocks run (a free waits for the whole device).
+      std::vector<std::vector<int>> cuts;
+      size_t ws_max = 0, nj_max = 0, tasks_max = 0, scratch_max = 0;
+      while ( pos < lr.size() ) {
+         std::vector<int> idx;
+         size_t bytes = 0, tasks = 0, scratch = 0;
+         while ( pos < lr.size() ) {
+            const Blk& b = h.blks[ lr[ pos ] ];
+            const size_t need = need_of( b );
+            if ( !idx.empty() && bytes + need > opt.ws_bytes ) break;
+            bytes += ( need + 255 ) / 256 * 256;
+            tasks += ( b.nr + kTrApply - 1 ) / kTrApply + ( b.nc + kTrApply - 1 ) / kTrApply;
+            const size_t kp = ( std::min( kcap_of( b ), kTrMaxK ) + 3 ) / 4 * 4;
+            scratch += 5 * kp * kp + 2 * kp;
+            idx.push_back( (int)pos++ );
+         }
+         ws_max = std::max( ws_max, bytes );
+         nj_max = std::max( nj_max, idx.size() );
+         tasks_max = std::max( tasks_max, tasks );
+         scratch_max = std::max( scratch_max, scratch );
+         cuts.push_back( std::move( idx ) );
+      }
+      Batch B;
+      if ( !cuts.empty() ) {
+         B.ws.ensure( ws_max );
+         B.jobs_d.ensure( nj_max );
+         B.rank_d.ensure( nj_max );
+         B.uo_d.ensure( nj_max );
+         B.vo_d.ensure( nj_max );
+         B.tj_d.ensure( nj_max );
+         B.trank_d.ensure( nj_max );
+         B.tfail_d.ensure( nj_max );
+         B.tuo_d.ensure( nj_max );
+         B.tvo_d.ensure( nj_max );
+         B.ttask_d.ensure( std::max<size_t>( tasks_max, 1 ) );
+         if ( opt.trunc ) B.tr_ws.ensure( scratch_max );
+      }
+      for ( auto& idx : cuts ) {
+         B.idx = std::move( idx );
+         launch( B, st, st_aux );
+         finish( B, st, st_aux );
+      }
+      if ( !big.idx.empty() ) {
+         finish( big, st_big, st_aux );
+         BH_HIP( hipEventRecord( ev, st_big ) );
+         BH_HIP( hipStreamWaitEvent( st, ev, 0 ) );
       }
    }
 

@@ -292,7 +292,7 @@ void hm_factor( bhg_solver* s ) {
    in.d_gdef = s->gdef.p;
    in.ndef = s->ndef;
    std::vector<std::string> lines;
-   s->hm = hm_build( s->st, in, s->hm_opt, &s->hm_stats, &lines );
+   s->hm = hm_build( s->st, s->st2, s->st3, in, s->hm_opt, &s->hm_stats, &lines );
    for ( auto& l : lines ) s->log.push_back( std::move( l ) );
    s->factored = -1;
    s->pivoted = false;
