@@ -272,7 +272,7 @@ __global__ void __launch_bounds__( 1024 ) k_deflation_weights( int n, int ndef, 
 // (columns 3 n2 .. 3 n2 + 2), so a wave's stores cover 192 contiguous doubles of each column.
 // The source record is uniform across the wave and loads as scalars.
 // deflate: 0 none; 1 add gdef[comp(n2)] xmu(j) xmu(i) where comp(n1) == comp(n2).
-__global__ void __launch_bounds__( 256 ) k_pairs( int n, int strip, double cg, const double* __restrict__ src,
+__global__ void __launch_bounds__( 256, 4 ) k_pairs( int n, int strip, double cg, const double* __restrict__ src,
                                                  const double* __restrict__ xc, const double* __restrict__ selfb,
                                                  const double* __restrict__ xmu, const int* __restrict__ comp,
                                                  const double* __restrict__ gdef, int deflate, double* __restrict__ gr,
