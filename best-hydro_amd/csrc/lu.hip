@@ -823,9 +823,18 @@ __global__ void __launch_bounds__( 256 ) k_resid_part( int npad, int chunk, int 
          if ( ok[ m ] && r < nrhs ) out[ (long)r * npad + row0 + 256 * m ] = acc[ m ][ r ];
 }
 
-__device__ inline void atomic_max_nonneg( double* addr, double v ) {
-   // Non-negative doubles order as their bit patterns do.
-   atomicMax( reinterpret_cast<unsigned long long*>( addr ), (unsigned long long)__double_as_longlong( v ) );
+// *addr = max(*addr, v) over a whole wave for non-negative doubles, which order as their bit
+// patterns do (every lane must call it): the largest bit pattern by shuffles, then one atomic per
+// wave.  A NaN (sign bit clear, as fabs leaves it) orders above every finite value and infinity,
+// so it still reaches *addr.
+__device__ inline void wave_atomic_max_nonneg( double* addr, double v ) {
+   unsigned long long b = (unsigned long long)__double_as_longlong( v );
+#pragma unroll
+   for ( int o = 32; o > 0; o >>= 1 ) {
+      const unsigned long long c = __shfl_xor( b, o );
+      b = c > b ? c : b;
+   }
+   if ( ( threadIdx.x & 63 ) == 0 ) atomicMax( reinterpret_cast<unsigned long long*>( addr ), b );
 }
 
 // r = b - sum of the chunks (in chunk order); norms[r] = max |r(:, rhs)| over rows < n3.
@@ -833,13 +842,16 @@ __global__ void __launch_bounds__( 256 ) k_resid_final( int npad, int n3, int nc
                                                        const double* __restrict__ part, const double* __restrict__ b,
                                                        long ldb, double* __restrict__ r, long ldr, double* norms ) {
    const int row = blockIdx.x * 256 + threadIdx.x;
-   if ( row >= npad ) return;
+   const bool in = row < npad;  // no early return: the wave reduction needs every lane
    for ( int k = 0; k < nrhs; ++k ) {
-      double s = 0.0;
-      for ( int c = 0; c < nchunks; ++c ) s = s + part[ (long)c * 16 * npad + (long)k * npad + row ];
-      const double v = row < n3 ? b[ (long)k * ldb + row ] - s : 0.0;
-      r[ (long)k * ldr + row ] = v;
-      atomic_max_nonneg( &norms[ k ], fabs( v ) );
+      double v = 0.0;
+      if ( in ) {
+         double s = 0.0;
+         for ( int c = 0; c < nchunks; ++c ) s = s + part[ (long)c * 16 * npad + (long)k * npad + row ];
+         v = row < n3 ? b[ (long)k * ldb + row ] - s : 0.0;
+         r[ (long)k * ldr + row ] = v;
+      }
+      wave_atomic_max_nonneg( &norms[ k ], fabs( v ) );
    }
 }
 
@@ -847,13 +859,16 @@ __global__ void __launch_bounds__( 256 ) k_resid_final( int npad, int n3, int nc
 __global__ void __launch_bounds__( 256 ) k_ir_update( int n3, int nrhs, double* __restrict__ x, long ldx,
                                                      const float* __restrict__ d, long ldd, double* norms ) {
    const int row = blockIdx.x * 256 + threadIdx.x;
-   if ( row >= n3 ) return;
+   const bool in = row < n3;  // no early return: the wave reductions need every lane
    for ( int k = 0; k < nrhs; ++k ) {
-      const double dv = (double)d[ (long)row * 16 + k ];
-      const double xv = x[ (long)k * ldx + row ] + dv;
-      x[ (long)k * ldx + row ] = xv;
-      atomic_max_nonneg( &norms[ k ], fabs( dv ) );
-      atomic_max_nonneg( &norms[ 16 + k ], fabs( xv ) );
+      double dv = 0.0, xv = 0.0;
+      if ( in ) {
+         dv = (double)d[ (long)row * 16 + k ];
+         xv = x[ (long)k * ldx + row ] + dv;
+         x[ (long)k * ldx + row ] = xv;
+      }
+      wave_atomic_max_nonneg( &norms[ k ], fabs( dv ) );
+      wave_atomic_max_nonneg( &norms[ 16 + k ], fabs( xv ) );
    }
 }
 
