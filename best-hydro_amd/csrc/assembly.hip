@@ -272,11 +272,14 @@ __global__ void __launch_bounds__( 1024 ) k_deflation_weights( int n, int ndef, 
 // (columns 3 n2 .. 3 n2 + 2), so a wave's stores cover 192 contiguous doubles of each column.
 // The source record is uniform across the wave and loads as scalars.
 // deflate: 0 none; 1 add gdef[comp(n2)] xmu(j) xmu(i) where comp(n1) == comp(n2).
+// F32: the entries are stored a second time, rounded to FP32, into gr32 (same layout), the copy
+// the FP32 factorization consumes.
+template <bool F32>
 __global__ void __launch_bounds__( 256, 4 ) k_pairs( int n, int strip, double cg, const double* __restrict__ src,
                                                  const double* __restrict__ xc, const double* __restrict__ selfb,
                                                  const double* __restrict__ xmu, const int* __restrict__ comp,
                                                  const double* __restrict__ gdef, int deflate, double* __restrict__ gr,
-                                                 long ld ) {
+                                                 float* __restrict__ gr32, long ld ) {
    const int n1 = blockIdx.x * blockDim.x + threadIdx.x;
    const int n2lo = blockIdx.y * strip;
    const int n2hi = min( n, n2lo + strip );
@@ -318,12 +321,22 @@ __global__ void __launch_bounds__( 256, 4 ) k_pairs( int n, int strip, double cg
          col[ (size_t)b * ld + 1 ] = gm[ 3 + b ];
          col[ (size_t)b * ld + 2 ] = gm[ 6 + b ];
       }
+      if constexpr ( F32 ) {
+         float* col32 = gr32 + 3 * (size_t)n1 + (size_t)( 3 * n2 ) * ld;
+#pragma unroll
+         for ( int b = 0; b < 3; ++b ) {
+            col32[ (size_t)b * ld + 0 ] = (float)gm[ b ];
+            col32[ (size_t)b * ld + 1 ] = (float)gm[ 3 + b ];
+            col32[ (size_t)b * ld + 2 ] = (float)gm[ 6 + b ];
+         }
+      }
    }
 }
 
 // Rows and columns n3 .. npad-1 of a padded matrix: identity, so the padded LU is
 // block-diagonal with the true factors in the leading block.
-__global__ void k_pad_identity( int n3, int npad, double* __restrict__ a, long ld ) {
+template <typename T>
+__global__ void k_pad_identity( int n3, int npad, T* __restrict__ a, long ld ) {
    const long total = (long)npad * ( npad - n3 ) * 2;
    for ( long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x ) {
       long i, j;
@@ -336,7 +349,7 @@ __global__ void k_pad_identity( int n3, int npad, double* __restrict__ a, long l
          i = n3 + u % ( npad - n3 );
          if ( j >= n3 ) continue;
       }
-      a[ j * ld + i ] = ( i == j ) ? 1.0 : 0.0;
+      a[ j * ld + i ] = ( i == j ) ? T( 1 ) : T( 0 );
    }
 }
 
@@ -387,15 +400,24 @@ void launch_deflation_weights( hipStream_t st, int n, int ndef, const double* se
 }
 
 void launch_pairs( hipStream_t st, int n, int strip, double cg, const double* src, const double* xc, const double* selfb,
-                   const double* xmu, const int* comp, const double* gdef, int deflate, double* gr, long ld ) {
+                   const double* xmu, const int* comp, const double* gdef, int deflate, double* gr, long ld, float* gr32 ) {
    dim3 grid( ( n + 255 ) / 256, ( n + strip - 1 ) / strip );
    // 3,457 flops per pair (3,120 add/mul + 106 sqrt + 210 div + 21 log, special functions as 1).
    ProfScope prof( kKPairs, 3457.0 * n * ( n - 1.0 ) );
-   hipLaunchKernelGGL( k_pairs, grid, dim3( 256 ), 0, st, n, strip, cg, src, xc, selfb, xmu, comp, gdef, deflate, gr, ld );
+   if ( gr32 )
+      hipLaunchKernelGGL( k_pairs<true>, grid, dim3( 256 ), 0, st, n, strip, cg, src, xc, selfb, xmu, comp, gdef, deflate, gr,
+                          gr32, ld );
+   else
+      hipLaunchKernelGGL( k_pairs<false>, grid, dim3( 256 ), 0, st, n, strip, cg, src, xc, selfb, xmu, comp, gdef, deflate, gr,
+                          nullptr, ld );
 }
 
 void launch_pad_identity( hipStream_t st, int n3, int npad, double* a, long ld ) {
-   if ( npad > n3 ) hipLaunchKernelGGL( k_pad_identity, dim3( 1024 ), dim3( 256 ), 0, st, n3, npad, a, ld );
+   if ( npad > n3 ) hipLaunchKernelGGL( k_pad_identity<double>, dim3( 1024 ), dim3( 256 ), 0, st, n3, npad, a, ld );
+}
+
+void launch_pad_identity32( hipStream_t st, int n3, int npad, float* a, long ld ) {
+   if ( npad > n3 ) hipLaunchKernelGGL( k_pad_identity<float>, dim3( 1024 ), dim3( 256 ), 0, st, n3, npad, a, ld );
 }
 
 void launch_entries( hipStream_t st, int n, double cg, const double* src, const double* xc, const double* selfb, int i0,

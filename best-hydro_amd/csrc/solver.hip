@@ -33,8 +33,9 @@ void launch_self( hipStream_t, int, const double*, double, const double*, const 
                   double*, double*, double*, double* );
 void launch_deflation_weights( hipStream_t, int, int, const double*, const double*, const int*, double*, double*, int );
 void launch_pairs( hipStream_t, int, int, double, const double*, const double*, const double*, const double*, const int*,
-                   const double*, int, double*, long );
+                   const double*, int, double*, long, float* gr32 = nullptr );
 void launch_pad_identity( hipStream_t, int, int, double*, long );
+void launch_pad_identity32( hipStream_t, int, int, float*, long );
 void launch_entries( hipStream_t, int, double, const double*, const double*, const double*, int, int, int, int, double*,
                      long );
 template <typename T>
@@ -126,6 +127,7 @@ struct bhg_solver {
    bhg::HMatrixPtr hm;
    bool pivoted = false;      // the slow path ran (rows were exchanged)
    bool deflated = true;      // the factored matrix carries the deflation
+   bool a32_fresh = false;    // a32 holds the FP32 rounding of the assembled a64 (the pair kernel stored both)
    bool transposed = false;   // the factors (and the FP64 matrix) are transposed in place: solves are with GR^T
    // vectors
    bhg::DevBuf<double> b, x, r, y64, x64il, p64, part, norms, sums;
@@ -170,6 +172,16 @@ void fill( hipStream_t st, int* p, long n, int v ) {
    hipLaunchKernelGGL( k_fill_i, dim3( (unsigned)std::min<long>( 2048, ( n + 255 ) / 256 ) ), dim3( 256 ), 0, st, n, p, v );
 }
 
+// The refinement's norms: the last step's 32 values move to norms[32..64) and norms[0..32) is
+// cleared for the next step.
+__global__ void k_ir_shift( double* norms ) {
+   const int k = threadIdx.x;
+   if ( k < 32 ) {
+      norms[ 32 + k ] = norms[ k ];
+      norms[ k ] = 0.0;
+   }
+}
+
 void set_device( bhg_solver* s ) { BH_HIP( hipSetDevice( s->device ) ); }
 
 // Sizes the per-mesh buffers for n triangles.
@@ -206,13 +218,18 @@ void self_terms( bhg_solver* s, bool weights, bool exact_order ) {
    }
 }
 
-void assemble( bhg_solver* s, bool deflate ) {
+// also32: the pair kernel also stores the FP32 rounding of every entry into a32, the copy the FP32
+// factorization consumes (so it needs no conversion pass).
+void assemble( bhg_solver* s, bool deflate, bool also32 = false ) {
    const size_t np = s->npad;
    s->a64.ensure( np * np );
+   if ( also32 ) s->a32.ensure( np * np );
    launch_pairs( s->st, s->n, kPairStrip, s->cg, s->src.p, s->xc.p, s->selfb.p, s->xmu.p, s->comp.p, s->gdef.p, deflate ? 1 : 0,
-                 s->a64.p, (long)np );
+                 s->a64.p, (long)np, also32 ? s->a32.p : nullptr );
    launch_pad_identity( s->st, s->n3, s->npad, s->a64.p, (long)np );
+   if ( also32 ) launch_pad_identity32( s->st, s->n3, s->npad, s->a32.p, (long)np );
    s->deflated = deflate;
+   s->a32_fresh = also32;
 }
 
 int read_flag( bhg_solver* s ) {
@@ -233,6 +250,8 @@ void factor( bhg_solver* s, int precision ) {
    fill( s->st, s->flags.p, 4, 0 );
    s->pivoted = false;
    s->transposed = false;
+   const bool fresh32 = s->a32_fresh;
+   s->a32_fresh = false;
    if ( precision == 64 ) {
       s->inv64.ensure( (size_t)nblk * 2 * kNB * kNB );
       lu_nopiv<double>( s->lu_streams(), s->npad, s->a64.p, (long)np, s->inv64.p, s->flags.p );
@@ -250,7 +269,7 @@ void factor( bhg_solver* s, int precision ) {
    } else {
       s->a32.ensure( np * np );
       s->inv32.ensure( (size_t)nblk * 2 * kNB * kNB );
-      convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
+      if ( !fresh32 ) convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
       g_growth = kFp32Growth;
       lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
       g_growth = 1.0;
@@ -314,7 +333,7 @@ void size_vectors( bhg_solver* s ) {
    s->x32.ensure( need );
    s->p32.ensure( need * 4 );
    s->part.ensure( residual_part_elems( s->npad ) );
-   s->norms.ensure( 32 );
+   s->norms.ensure( 64 );
    s->sums.ensure( (size_t)kSumGroups * kSumSlots );
 
    if ( grow ) {
@@ -327,10 +346,11 @@ void size_vectors( bhg_solver* s ) {
 int piv_mode( const bhg_solver* s ) { return s->pivoted ? ( s->transposed ? -1 : 1 ) : 0; }
 
 // One refinement step of the FP32 path: x0 = LU32^-1 b (it == 0), else r = b - A x in FP64 and
-// x += LU32^-1 r; norms[k] = max |dx|, norms[16 + k] = max |x| per column.  No host interaction.
+// x += LU32^-1 r; norms[k] = max |dx|, norms[16 + k] = max |x| per column, norms[32 .. 64) the
+// previous step's.  No host interaction.
 void ir_step( bhg_solver* s, int nrhs, int it ) {
    const long np = s->npad;
-   fill( s->st, s->norms.p, 32, 0.0 );
+   hipLaunchKernelGGL( k_ir_shift, dim3( 1 ), dim3( 64 ), 0, s->st, s->norms.p );
    if ( it == 0 ) {
       fill( s->st, s->x.p, np * nrhs, 0.0 );
       to_interleaved<double, float>( s->st, s->npad, nrhs, s->b.p, np, s->d32.p );
@@ -344,33 +364,48 @@ void ir_step( bhg_solver* s, int nrhs, int it ) {
    ir_update( s->st, s->n3, nrhs, s->x.p, np, s->x32.p, np, s->norms.p );
 }
 
-// max over the columns of |dx| / |x| from the last step's norms (synchronizes).
-double ir_correction( bhg_solver* s, int nrhs ) {
-   double hn[ 32 ];
+// The last refinement step, per column k: the correction c_k = max |dx| / max |x| and, from the
+// step before (correction p_k), the error left in x estimated as c_k (c_k / p_k): refinement
+// converges linearly, each step's correction being the error of the x before it, so c_k / p_k
+// estimates the contraction.  Columns that do not contract by half or more get no estimate.
+struct IrCorrection {
+   double worst = 0.0;  // max_k c_k
+   double est = 0.0;    // max_k c_k^2 / p_k (infinite without a contraction estimate)
+};
+IrCorrection ir_correction( bhg_solver* s, int nrhs ) {
+   double hn[ 64 ];
    BH_HIP( hipMemcpyAsync( hn, s->norms.p, sizeof hn, hipMemcpyDeviceToHost, s->st ) );
    BH_HIP( hipStreamSynchronize( s->st ) );
-   double worst = 0.0;
+   IrCorrection r;
    for ( int k = 0; k < nrhs; ++k ) {
-      const double rel = hn[ 16 + k ] > 0 ? hn[ k ] / hn[ 16 + k ] : 0.0;
-      if ( !std::isfinite( rel ) ) throw std::runtime_error( "iterative refinement produced a non-finite correction" );
-      worst = std::max( worst, rel );
+      const double c = hn[ 16 + k ] > 0 ? hn[ k ] / hn[ 16 + k ] : 0.0;
+      const double p = hn[ 48 + k ] > 0 ? hn[ 32 + k ] / hn[ 48 + k ] : 0.0;
+      if ( !std::isfinite( c ) ) throw std::runtime_error( "iterative refinement produced a non-finite correction" );
+      r.worst = std::max( r.worst, c );
+      const double e = c == 0.0 ? 0.0 : ( p > 0.0 && c < 0.5 * p ) ? c * ( c / p ) : HUGE_VAL;
+      r.est = std::max( r.est, e );
    }
-   return worst;
+   return r;
 }
 
-// FP64 iterative refinement with the FP32 factors from step it0 on, until the correction is below
-// ir_tol relative to x in every column (or stagnates at the FP64 floor).
-// Returns false if the correction stagnated above kIrFloor (the FP32 factors too poor for the
-// refinement to converge).
+// Converged after step it: the last correction, or the error it leaves estimated from the two
+// last corrections, below tol (step 0's correction is 1, the whole of x0).
+bool ir_converged( const IrCorrection& c, int it, double tol ) { return it > 0 && ( c.worst < tol || c.est < tol ); }
+
+// FP64 iterative refinement with the FP32 factors from step it0 on, until the correction, or the
+// error estimated to remain, is below ir_tol relative to x in every column (or the correction
+// stagnates at the FP64 floor).  Returns false if the correction stagnated above kIrFloor (the
+// FP32 factors too poor for the refinement to converge).
 constexpr double kIrFloor = 1e-10;
 bool ir_loop( bhg_solver* s, int nrhs, int it0 ) {
    double prev = 1e300, worst = 1e300;
    for ( int it = it0; it < s->ir_maxit; ++it ) {
       ir_step( s, nrhs, it );
-      worst = ir_correction( s, nrhs );
+      const IrCorrection c = ir_correction( s, nrhs );
+      worst = c.worst;
       s->stats.ir_iterations = it + 1;
       s->stats.ir_correction = worst;
-      if ( it > 0 && worst < s->ir_tol ) return true;
+      if ( ir_converged( c, it, s->ir_tol ) ) return true;
       if ( it > 2 && worst > 0.5 * prev ) break;  // stagnation
       prev = worst;
    }
@@ -513,11 +548,13 @@ void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshT
    BH_HIP( hipMemcpyAsync( s->v.p, s->raw_v.p, 9 * (size_t)n * sizeof( double ), hipMemcpyDeviceToDevice, s->st ) );
    launch_prep( s->st, n, V1( s ), V2( s ), V3( s ), iarea, s->truearea, s->area.p, s->scal.p );
    self_terms( s, true, false );
-   launch_pairs( s->st, n, t.strip, s->cg, s->src.p, s->xc.p, s->selfb.p, s->xmu.p, s->comp.p, s->gdef.p, 1, s->a64.p, np );
+   launch_pairs( s->st, n, t.strip, s->cg, s->src.p, s->xc.p, s->selfb.p, s->xmu.p, s->comp.p, s->gdef.p, 1, s->a64.p, np,
+                 s->a32.p );
    launch_pad_identity( s->st, s->n3, s->npad, s->a64.p, np );
+   launch_pad_identity32( s->st, s->n3, s->npad, s->a32.p, np );
    s->deflated = true;
+   s->a32_fresh = false;
    fill( s->st, s->flags.p, 4, 0 );
-   convert_d2f( s->st, np * np, s->a64.p, s->a32.p );
    g_growth = kFp32Growth;
    lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
    g_growth = 1.0;
@@ -539,15 +576,15 @@ void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
    s->mesh_ready = true;
    if ( read_flag( s ) ) {
       s->say( "series: a multiplier exceeded 256; the mesh is solved again with row exchanges" );
-      assemble( s, true );
+      assemble( s, true, true );
       factor( s, 32 );
       launch_rhs( s->st, s->n, s->xc.p, s->b.p, s->npad, ivisco ? 1 : 0 );
       solve_device( s, nrhs );
    } else {
-      const double c = ir_correction( s, nrhs );
+      const IrCorrection c = ir_correction( s, nrhs );
       s->stats.ir_iterations = t.ir_iters;
-      s->stats.ir_correction = c;
-      if ( !( c < s->ir_tol ) && !ir_loop( s, nrhs, t.ir_iters ) ) {
+      s->stats.ir_correction = c.worst;
+      if ( !ir_converged( c, t.ir_iters - 1, s->ir_tol ) && !ir_loop( s, nrhs, t.ir_iters ) ) {
          refactor_pivoted32( s );
          if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
       }
@@ -582,7 +619,7 @@ void solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int 
       T.mark( s->st, 2 );
       hm_factor( s );
    } else {
-      assemble( s, true );
+      assemble( s, true, precision == 32 );
       T.mark( s->st, 2 );
       factor( s, precision );
    }
@@ -1262,7 +1299,7 @@ int bhg_factor_deflated( bhg_solver* s, int ndef, const double* gdef, const int*
          s->gdef.ensure( s->ndef );
          bhg::launch_deflation_weights( s->st, s->n, s->ndef, s->selfb.p, s->xmu.p, s->comp.p, s->gdef.p, s->scal.p + 5, 1 );
       }
-      bhg::assemble( s, true );
+      bhg::assemble( s, true, precision == 32 );
       bhg::factor( s, precision );
       bhg::size_vectors( s );
       BH_HIP( hipStreamSynchronize( s->st ) );
@@ -1483,7 +1520,7 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
          for ( int strip : { 16, 32, 64, 128 } ) {
             const double ms = time( [&] {
                bhg::launch_pairs( m->st, m->n, strip, m->cg, m->src.p, m->xc.p, m->selfb.p, m->xmu.p, m->comp.p, m->gdef.p, 1,
-                                  m->a64.p, np );
+                                  m->a64.p, np, m->a32.p );
             } );
             if ( ms < tb ) tb = ms, t.strip = strip;
          }
