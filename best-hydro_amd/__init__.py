@@ -408,7 +408,7 @@ class Series:
         self._check(self.L.bhg_series_autotune(self.h, int(iarea), float(cg), int(ivisco)), "autotune")
         return self.tuning()
 
-    def set_tuning(self, k, strip=64, gemm_bk=32, ir_iters=4):
+    def set_tuning(self, k, strip=64, gemm_bk=0, ir_iters=3):
         self._check(self.L.bhg_series_set_tuning(self.h, int(k), int(strip), int(gemm_bk), int(ir_iters)), "set_tuning")
 
     def tuning(self):

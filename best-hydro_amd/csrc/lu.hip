@@ -60,8 +60,16 @@ struct BatchStride {
    long hi = 0, lo = 0;
 };
 
+// Waves per SIMD the register allocation must allow: three for FP32 with 16-column slices (LDS then
+// holds three workgroups per CU, and the accumulators move to ordinary VGPRs without spills),
+// else no constraint.
+template <typename T, int BK>
+struct GemmOcc {
+   static constexpr int waves = ( sizeof( T ) == 4 && BK == 16 ) ? 3 : 1;
+};
+
 template <typename T, int BK, int BM = 128, int BN = 128>
-__global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
+__global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
                                                 T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
                                                 BatchStride sb, BatchStride sc ) {
    constexpr int PAD = 8;
@@ -119,16 +127,6 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
    // reads C in the epilogue.
    constexpr bool kPre = sizeof( T ) == 4;
    T cpre[ TM ][ TN ][ 4 ];
-   if ( kPre && beta != T( 0 ) ) {
-#pragma unroll
-      for ( int t = 0; t < TM; ++t )
-#pragma unroll
-         for ( int u = 0; u < TN; ++u ) {
-            const T* c = C + (long)( bn * BN + wn * WN + u * 16 + li ) * ldc + bm * BM;
-#pragma unroll
-            for ( int r = 0; r < 4; ++r ) cpre[ t ][ u ][ r ] = c[ crow( t, r ) ];
-         }
-   }
    const int nk = K / BK;
    auto gload = [&]( int kt ) {
       const T* pa = Ab + (long)( kt * BK + a_k ) * lda + a_m;
@@ -145,6 +143,17 @@ __global__ void __launch_bounds__( 256 ) k_gemm( int M, int N, int K, T alpha, c
       for ( int e = 0; e < BE; ++e ) Bs[ stg ][ b_k + e ][ b_slot ] = rb[ e ];
    };
    gload( 0 );
+   // the C tile after the first slice, so that storing the slice to LDS need not wait for it
+   if ( kPre && beta != T( 0 ) ) {
+#pragma unroll
+      for ( int t = 0; t < TM; ++t )
+#pragma unroll
+         for ( int u = 0; u < TN; ++u ) {
+            const T* c = C + (long)( bn * BN + wn * WN + u * 16 + li ) * ldc + bm * BM;
+#pragma unroll
+            for ( int r = 0; r < 4; ++r ) cpre[ t ][ u ][ r ] = c[ crow( t, r ) ];
+         }
+   }
    sstore( 0 );
    __syncthreads();
    for ( int kt = 0; kt < nk; ++kt ) {
@@ -882,9 +891,10 @@ __global__ void k_f2d( long n, const float* __restrict__ a, double* __restrict__
 
 // ------------------------------------------------------------------ host side
 
-// K slice of the FP32 GEMM (16, 32 or 64), chosen per mesh by the series autotuner; every launch
-// made on this host thread uses it.
-thread_local int g_gemm_bk_f32 = 32;
+// K slice of the FP32 GEMM: 16, 32 or 64 for every tile shape, or 0 for 16 on the 128 x 128 tiles
+// of the trailing updates (three workgroups per CU) and 32 on the smaller ones.  Chosen per mesh
+// by the series autotuner; every launch made on this host thread uses it.
+thread_local int g_gemm_bk_f32 = 0;
 
 // The largest multiplier the exchange-free factorization accepts before the caller reruns it with
 // row exchanges: 1 is partial pivoting's own test (the factors are then DGETRF's); the FP32
@@ -898,7 +908,7 @@ void launch_gemm_shape( hipStream_t st, int M, int N, int K, T alpha, const T* A
    if constexpr ( sizeof( T ) == 8 )
       hipLaunchKernelGGL( ( k_gemm<double, 16, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                           check, T( g_growth ), zdiv, sa, sb, sc );
-   else if ( g_gemm_bk_f32 == 16 || K % 32 != 0 )
+   else if ( g_gemm_bk_f32 == 16 || K % 32 != 0 || ( g_gemm_bk_f32 == 0 && BM == 128 && BN == 128 ) )
       hipLaunchKernelGGL( ( k_gemm<float, 16, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                           check, T( g_growth ), zdiv, sa, sb, sc );
    else if ( g_gemm_bk_f32 == 64 && K % 64 == 0 )

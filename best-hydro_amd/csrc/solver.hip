@@ -515,7 +515,7 @@ void compute_results( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
 // Tile choices of one mesh of a series (picked by the autotuner).
 struct MeshTune {
    int strip = kPairStrip;  // source triangles per workgroup of the pair kernel
-   int gemm_bk = 32;        // K slice of the FP32 GEMM
+   int gemm_bk = 0;         // K slice of the FP32 GEMM (0: 16 on the 128 x 128 tiles, 32 on the others)
    int ir_iters = 4;        // refinement steps in the captured schedule
 };
 
@@ -562,7 +562,7 @@ void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshT
    launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
    const int nrhs = ivisco ? 11 : 6;
    for ( int it = 0; it < t.ir_iters; ++it ) ir_step( s, nrhs, it );
-   g_gemm_bk_f32 = 32;
+   g_gemm_bk_f32 = 0;
 }
 
 // After an enqueued solve has run: if partial pivoting would have exchanged rows, the mesh is
@@ -1466,8 +1466,8 @@ int bhg_series_set_mesh( bhg_series* S, int k, int n, const double* v1, const do
 int bhg_series_set_tuning( bhg_series* S, int k, int strip, int gemm_bk, int ir_iters ) {
    return bhg::series_guarded( S, "bhg_series_set_tuning", [&] {
       if ( k < 0 || k >= (int)S->m.size() ) throw std::runtime_error( "mesh index out of range" );
-      if ( strip < 1 || ( gemm_bk != 16 && gemm_bk != 32 && gemm_bk != 64 ) || ir_iters < 1 )
-         throw std::runtime_error( "strip >= 1, gemm_bk 16, 32 or 64, ir_iters >= 1" );
+      if ( strip < 1 || ( gemm_bk != 0 && gemm_bk != 16 && gemm_bk != 32 && gemm_bk != 64 ) || ir_iters < 1 )
+         throw std::runtime_error( "strip >= 1, gemm_bk 0, 16, 32 or 64, ir_iters >= 1" );
       S->tune[ k ] = bhg::MeshTune { strip, gemm_bk, ir_iters };
       S->drop_graph();
    } );
@@ -1525,6 +1525,8 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
             if ( ms < tb ) tb = ms, t.strip = strip;
          }
          tb = 1e300;
+         // one K slice for every tile shape: the per-shape default (0) times fastest for a mesh
+         // alone but slowed the side-by-side schedule of the series (352 -> 370 ms)
          for ( int bk : { 16, 32, 64 } ) {
             const double ms = time( [&] {
                bhg::g_gemm_bk_f32 = bk;
@@ -1535,22 +1537,31 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
                bhg::g_growth = 1.0;
                bhg::build_super_inverses<float>( m->st, m->npad, bhg::kSuper, m->a32.p, np, m->inv32.p, m->xl32.p, m->xu32.p,
                                                  m->tmp32.p );
-               bhg::g_gemm_bk_f32 = 32;
+               bhg::g_gemm_bk_f32 = 0;
             } );
             if ( ms < tb ) tb = ms, t.gemm_bk = bk;
          }
          S->tune[ k ] = t;
       }
-      // side by side or one after another: the faster schedule
+      // The schedule, timed as a whole: side by side or one after another, each with the K slices
+      // chosen per mesh or with 32 for every mesh (a slice that is fastest for a mesh alone can be
+      // slower when the meshes share the GPU).
+      const std::vector<bhg::MeshTune> tuned = S->tune;
+      std::vector<bhg::MeshTune> k32 = tuned;
+      for ( auto& t : k32 ) t.gemm_bk = 32;
       double best = 1e300;
-      int choice = 1;
-      for ( int par : { 1, 0 } ) {
-         bhg::series_capture( S, iarea, cg, ivisco, par );
-         bhg::series_replay( S );
-         const double ms = bhg::series_replay( S );
-         if ( ms < best ) best = ms, choice = par;
+      int choice = 1, choice_k = 0;
+      for ( int kset : { 0, 1 } ) {
+         S->tune = kset ? k32 : tuned;
+         for ( int par : { 1, 0 } ) {
+            bhg::series_capture( S, iarea, cg, ivisco, par );
+            bhg::series_replay( S );
+            const double ms = bhg::series_replay( S );
+            if ( ms < best ) best = ms, choice = par, choice_k = kset;
+         }
       }
-      if ( choice != S->parallel ) bhg::series_capture( S, iarea, cg, ivisco, choice );
+      S->tune = choice_k ? k32 : tuned;
+      bhg::series_capture( S, iarea, cg, ivisco, choice );
    } );
 }
 

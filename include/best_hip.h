@@ -228,9 +228,12 @@ int bhg_series_set_mesh( bhg_series* s, int k, int n, const double* v1, const do
 
 /* The autotuner: for each mesh, the pair kernel's strip of source triangles per workgroup (16, 32,
  * 64, 128) and the FP32 GEMM's K slice (16, 32, 64) that time fastest, the refinement steps an
- * eager solve needed; then whether the meshes run side by side or one after another in the
- * schedule, by timing both graphs.  Captures the schedule. */
+ * eager solve needed; then, by timing four graphs, whether the meshes run side by side or one
+ * after another and whether they keep those K slices or all take 32 (what is fastest for a mesh
+ * alone can be slower when the meshes share the GPU).  Captures the schedule. */
 int bhg_series_autotune( bhg_series* s, int iarea, double cg, int ivisco );
+/* gemm_bk: 16, 32 or 64 on every tile, or 0 (a single solve's default): 16 on the 128 x 128 tiles
+ * of the trailing updates and 32 on the smaller ones. */
 int bhg_series_set_tuning( bhg_series* s, int k, int strip, int gemm_bk, int ir_iters );
 /* out[0..3] = strip, gemm_bk, ir_iters, side by side (1) or chained (0). */
 int bhg_series_get_tuning( bhg_series* s, int k, int* out );
