@@ -366,6 +366,64 @@ __device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i ) {
    }
 }
 
+// Half of the columns of leaf_inv_lower / leaf_inv_upper, so that two waves share each inverse:
+// part 0 owns columns 0-7 and 24-31, part 1 columns 8-23, equal work (column c of L^-1 takes
+// 31 - c steps, of U^-1 c - 1).  Each wave holds the whole rows (the multipliers of every step)
+// and updates only its own columns; the column sets are compile-time, so no branches remain.
+template <int P>
+__device__ __forceinline__ constexpr bool leaf_owns( int c ) {
+   return P == 0 ? ( c < 8 || c >= 24 ) : ( c >= 8 && c < 24 );
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void leaf_inv_lower_part( T ( &a )[ 32 ], int i ) {
+#pragma unroll
+   for ( int k = 0; k < 32; ++k ) {
+      const T li = i > k ? a[ k ] : T( 0 );
+      T prow[ 32 ];
+#pragma unroll
+      for ( int c = 0; c < k; ++c )
+         if ( leaf_owns<P>( c ) ) prow[ c ] = bcast( a[ c ], k );
+#pragma unroll
+      for ( int c = 0; c < k; ++c )
+         if ( leaf_owns<P>( c ) ) a[ c ] = a[ c ] - li * prow[ c ];
+      if ( leaf_owns<P>( k ) ) a[ k ] = i > k ? -li : a[ k ];
+   }
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void leaf_inv_upper_part( T ( &a )[ 32 ], int i ) {
+   T dg = a[ 0 ];
+#pragma unroll
+   for ( int c = 1; c < 32; ++c ) dg = i == c ? a[ c ] : dg;
+   const T rd = T( 1 ) / dg;
+   // every column is scaled: the multipliers u1_ik of the other part's columns are needed too
+#pragma unroll
+   for ( int c = 1; c < 32; ++c ) a[ c ] = c > i ? a[ c ] * rd : a[ c ];
+#pragma unroll
+   for ( int k = 31; k > 0; --k ) {
+      const T ui = i < k ? a[ k ] : T( 0 );
+      T prow[ 32 ];
+#pragma unroll
+      for ( int c = k + 1; c < 32; ++c )
+         if ( leaf_owns<P>( c ) ) prow[ c ] = bcast( a[ c ], k );
+#pragma unroll
+      for ( int c = k + 1; c < 32; ++c )
+         if ( leaf_owns<P>( c ) ) a[ c ] = a[ c ] - ui * prow[ c ];
+      if ( leaf_owns<P>( k ) ) a[ k ] = i < k ? -ui : a[ k ];
+   }
+   T rdc[ 32 ];
+#pragma unroll
+   for ( int c = 0; c < 32; ++c )
+      if ( leaf_owns<P>( c ) ) rdc[ c ] = bcast( rd, c );
+#pragma unroll
+   for ( int c = 0; c < 32; ++c ) {
+      if ( !leaf_owns<P>( c ) ) continue;
+      const T x = i == c ? T( 1 ) : a[ c ];
+      a[ c ] = c >= i ? x * rdc[ c ] : a[ c ];
+   }
+}
+
 // The diagonal block at A (leading dimension lda): mode 0 factors it without exchanges (flagging
 // any |a_ij| > |a_jj| at the step that eliminates column j, and any multiplier above 1 in
 // magnitude) and then inverts both triangles; mode 1 only inverts an already-factored block.
@@ -422,21 +480,35 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
             }
          }
          __syncthreads();
-         if ( wave == 0 ) {
-            leaf_inv_lower( a, i );
+         // L^-1 on waves 0 and 1, U^-1 on waves 2 and 3, half of the columns each: every wave reads
+         // the factored tile's rows first (wave 0 still holds them), then all write their columns
+         if ( wave == 1 ) {
+#pragma unroll
+            for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
+         } else if ( wave >= 2 ) {
+#pragma unroll
+            for ( int c = 0; c < 32; ++c ) a[ c ] = c >= i ? Spp[ c * kLd + i ] : T( 0 );
+         }
+         __syncthreads();
+         if ( wave < 2 ) {
+            if ( wave == 0 )
+               leaf_inv_lower_part<0>( a, i );
+            else
+               leaf_inv_lower_part<1>( a, i );
             if ( lane < 32 ) {
 #pragma unroll
                for ( int c = 0; c < 31; ++c )
-                  if ( c < i ) Spp[ c * kLd + i ] = a[ c ];
+                  if ( c < i && ( wave == 0 ? leaf_owns<0>( c ) : leaf_owns<1>( c ) ) ) Spp[ c * kLd + i ] = a[ c ];
             }
-         } else if ( wave == 1 ) {
-#pragma unroll
-            for ( int c = 0; c < 32; ++c ) a[ c ] = c >= i ? Spp[ c * kLd + i ] : T( 0 );
-            leaf_inv_upper( a, i );
+         } else {
+            if ( wave == 2 )
+               leaf_inv_upper_part<0>( a, i );
+            else
+               leaf_inv_upper_part<1>( a, i );
             if ( lane < 32 ) {
 #pragma unroll
                for ( int c = 0; c < 32; ++c )
-                  if ( c >= i ) Spp[ c * kLd + i ] = a[ c ];
+                  if ( c >= i && ( wave == 2 ? leaf_owns<0>( c ) : leaf_owns<1>( c ) ) ) Spp[ c * kLd + i ] = a[ c ];
             }
          }
          __syncthreads();
