@@ -9,7 +9,9 @@ the DIFCO/VISCO reductions to D_t, D_r, the coupling tensor and CHI*V ([eta]).
 Prints one JSON line.  Besides the contract fields it carries the FP64-LU path's rate on the same
 mesh, the errors of both paths against the oracle (the reference's dense FP64 path restated in C
 with LAPACK), the roofline of the dominant kernel class (HIP events around every launch of the
-class during the timed steps), the oracle's CPU rate on this host, and the H-matrix path on the
+class during the timed steps), each phase of a solve against its own roofline (phase_rooflines:
+assembly on the FP64 VALU, the LU on the matrix cores, the solves and residuals on HBM), the
+oracle's CPU rate on this host, and the H-matrix path on the
 81,920-triangle sphere (hmatrix_c5: solves/s, build and GMRES times, and the HBM roofline of the
 H-matrix product), the series of eight meshes for the 1/N extrapolation solved by one captured
 graph launch (series_c3), and the adjoint gradient of D_t, D_r and CHI*V with respect to the
@@ -280,6 +282,34 @@ def main():
                 "profiled_ms_per_step": round(1e3 * elp / args.steps, 3),
                 "classes_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in prof32.items() if v["launches"]}}
 
+    # Each phase of one solve against the roofline that bounds it, from the solver's phase timers
+    # (HIP events) and the algorithmic work per unit: assembly 3,457 flops per triangle pair
+    # (special functions counted once) on the FP64 VALU; the LU 2/3 npad^3 on the matrix cores; the
+    # triangular solves and refinement residuals as the bytes of the factors and of GR they stream.
+    npad = st32["npad"]
+    lu_flops = 2.0 / 3.0 * npad ** 3
+    pair_flops = 3457.0 * n * (n - 1.0)
+    super_blk = 1024
+    f32_solve_bytes = 4.0 * (npad * npad + 2 * npad * super_blk)   # L, U and the super-block inverses
+    f64_solve_bytes = 8.0 * (npad * npad + 2 * npad * super_blk)
+    resid_bytes = 8.0 * npad * npad
+    nsolve = st32["ir_iterations"]
+
+    def phase(work, ms, peak, unit):
+        achieved = work / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
+        return {"achieved": round(achieved, 2), "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
+                "ms": round(ms, 3)}
+    phase_rooflines = {
+        "assembly_fp64_valu": phase(pair_flops, st32["t_assemble"], PEAKS["fp64_valu_tflops"], "TFLOP/s"),
+        "lu_fp32_mfma": phase(lu_flops, st32["t_factor"], PEAKS["fp32_mfma_tflops"], "TFLOP/s"),
+        "lu_fp64_mfma": phase(lu_flops, st64["t_factor"], PEAKS["fp64_mfma_tflops"], "TFLOP/s"),
+        "solves_and_refinement_hbm": phase(nsolve * f32_solve_bytes + (nsolve - 1) * resid_bytes, st32["t_solve"],
+                                           PEAKS["hbm_gbs"], "GB/s"),
+        "solve_fp64_hbm": phase(f64_solve_bytes, st64["t_solve"], PEAKS["hbm_gbs"], "GB/s"),
+        "note": "factor phases include the super-block inverses of the solves; the solve phase of the mixed "
+                f"path is {nsolve} FP32 triangular solves and {nsolve - 1} FP64 residual sweeps",
+    }
+
     out = {
         "metric": METRIC, "value": round(1e3 / ms32, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms32, 3), "higher_is_better": True, "scaling": "weak",
@@ -292,6 +322,7 @@ def main():
         "phases_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("t_") and not k.startswith("t_adjoint")},
         "ir_iterations": st32["ir_iterations"],
         "roofline": roofline,
+        "phase_rooflines": phase_rooflines,
     }
     if not args.no_cpu:
         ref, el = cpu_baseline(tri, cores)
