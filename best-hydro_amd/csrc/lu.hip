@@ -957,9 +957,6 @@ __global__ void k_d2f( long n, const double* __restrict__ a, float* __restrict__
    for ( long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x ) b[ i ] = (float)a[ i ];
 }
 
-__global__ void k_f2d( long n, const float* __restrict__ a, double* __restrict__ b ) {
-   for ( long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x ) b[ i ] = (double)a[ i ];
-}
 
 // ------------------------------------------------------------------ host side
 
@@ -1258,8 +1255,5 @@ void convert_d2f( hipStream_t st, long n, const double* a, float* b ) {
    hipLaunchKernelGGL( k_d2f, dim3( 2048 ), dim3( 256 ), 0, st, n, a, b );
 }
 
-void convert_f2d( hipStream_t st, long n, const float* a, double* b ) {
-   hipLaunchKernelGGL( k_f2d, dim3( 2048 ), dim3( 256 ), 0, st, n, a, b );
-}
 
 }  // namespace bhg

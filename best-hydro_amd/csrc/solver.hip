@@ -62,7 +62,6 @@ extern thread_local double g_growth;
 // the FP64 matrix removes the effect of the larger growth (the FP64 factorization keeps partial
 // pivoting's own test, 1, so its factors are DGETRF's).
 constexpr double kFp32Growth = 256.0;
-void convert_f2d( hipStream_t, long, const float*, double* );
 
 
 // Order of the diagonal super-blocks whose inverses the triangular solves use; the matrix is
