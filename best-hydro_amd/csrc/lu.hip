@@ -338,11 +338,10 @@ __device__ __forceinline__ void leaf_inv_lower( T ( &a )[ 32 ], int i ) {
 // untouched), as U1^-1 D^-1 with U = D U1: each row is scaled by its own 1 / u_ii, the unit upper
 // triangle U1 is inverted by rows from the bottom (for k = 31..0, rows i < k get X(i, c) -= u1_ik
 // X(k, c) for c > k and X(i, k) = -u1_ik), and column c of the result is scaled by 1 / u_cc.
+// dg is this row's diagonal entry u_ii, read by the caller from LDS: selecting it from a[] by the
+// lane's row index would make the compiler send a[] through scratch memory.
 template <typename T>
-__device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i ) {
-   T dg = a[ 0 ];
-#pragma unroll
-   for ( int c = 1; c < 32; ++c ) dg = i == c ? a[ c ] : dg;
+__device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i, T dg ) {
    const T rd = T( 1 ) / dg;
 #pragma unroll
    for ( int c = 1; c < 32; ++c ) a[ c ] = c > i ? a[ c ] * rd : a[ c ];
@@ -392,10 +391,7 @@ __device__ __forceinline__ void leaf_inv_lower_part( T ( &a )[ 32 ], int i ) {
 }
 
 template <int P, typename T>
-__device__ __forceinline__ void leaf_inv_upper_part( T ( &a )[ 32 ], int i ) {
-   T dg = a[ 0 ];
-#pragma unroll
-   for ( int c = 1; c < 32; ++c ) dg = i == c ? a[ c ] : dg;
+__device__ __forceinline__ void leaf_inv_upper_part( T ( &a )[ 32 ], int i, T dg ) {
    const T rd = T( 1 ) / dg;
    // every column is scaled: the multipliers u1_ik of the other part's columns are needed too
 #pragma unroll
@@ -489,6 +485,7 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
 #pragma unroll
             for ( int c = 0; c < 32; ++c ) a[ c ] = c >= i ? Spp[ c * kLd + i ] : T( 0 );
          }
+         const T dg = Spp[ i * kLd + i ];
          __syncthreads();
          if ( wave < 2 ) {
             if ( wave == 0 )
@@ -502,9 +499,9 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
             }
          } else {
             if ( wave == 2 )
-               leaf_inv_upper_part<0>( a, i );
+               leaf_inv_upper_part<0>( a, i, dg );
             else
-               leaf_inv_upper_part<1>( a, i );
+               leaf_inv_upper_part<1>( a, i, dg );
             if ( lane < 32 ) {
 #pragma unroll
                for ( int c = 0; c < 32; ++c )
@@ -557,7 +554,7 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
 #pragma unroll
       for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
       leaf_inv_lower( a, i );
-      leaf_inv_upper( a, i );
+      leaf_inv_upper( a, i, Spp[ i * kLd + i ] );
       if ( lane < 32 ) {
 #pragma unroll
          for ( int c = 0; c < 32; ++c ) Spp[ c * kLd + i ] = a[ c ];
