@@ -21,7 +21,7 @@ struct HmOptions {
    int single = 0;             // 1: blocks stored in FP32 (arithmetic in FP64)
    int trunc = 1;              // 1: approximations recompressed to eps by QR and SVD, as the reference does
    double tol = 0.0;           // GMRES relative residual; 0: aca_factor * eps
-   int restart = 30;           // GMRES restart length
+   int restart = 120;          // GMRES restart length (the reference's 30 with its H-LU; 120 with block Jacobi)
    int maxsteps = 500;         // GMRES steps (batched products) per solve
    double ws_bytes = 24e9;     // device work space of one batch of cross approximations
 };
