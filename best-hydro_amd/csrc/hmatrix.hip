@@ -902,6 +902,8 @@ struct BlkDev {
 
 constexpr int kSeg = 1024;  // columns of V per task of k_hm_vtx
 constexpr long kAcaBig = 6144;  // blocks with at least this many rows + columns: 512-thread approximation
+constexpr long kAcaSmall = 768;  // blocks with fewer rows + columns: one wave each (a 256-thread
+                                 // workgroup would leave most lanes idle in their row evaluations)
 constexpr int kDotChunkDefl = 8192;  // rows per partial sum of the deflation product
 constexpr int kBatch = 8;   // loads in flight per lane in the product kernels
 
@@ -1469,7 +1471,8 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
                 sizeof( int ) * ( b.nr / 3 + 2 ) + 256;
       };
       // Work space and jobs of the batch's blocks, then the approximations: the 512-thread prefix
-      // (the batch is in decreasing size) on s2, the rest on s (s2 == s: all on s).
+      // (the batch is in decreasing size) on s2, the rest on s (s2 == s: all on s), in 256-thread
+      // workgroups and, for the smallest blocks, one wave each.
       auto launch = [&]( Batch& B, hipStream_t s, hipStream_t s2 ) {
          B.jobs.clear();
          std::vector<size_t> offs;
@@ -1511,9 +1514,14 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
          if ( nbig > 0 )
             hipLaunchKernelGGL( k_hm_aca<512>, dim3( nbig ), dim3( 512 ), 0, s2, B.jobs_d.p, K, opt.aca_factor * opt.eps,
                                 B.rank_d.p, calls.p );
-         if ( nj > nbig )
-            hipLaunchKernelGGL( k_hm_aca<kWg>, dim3( nj - nbig ), dim3( kWg ), 0, s, B.jobs_d.p + nbig, K,
+         int nsm = nbig;  // the sizes decrease: the one-wave blocks are a suffix
+         while ( nsm < nj && 3L * ( B.jobs[ nsm ].m + B.jobs[ nsm ].nc ) >= kAcaSmall ) ++nsm;
+         if ( nsm > nbig )
+            hipLaunchKernelGGL( k_hm_aca<kWg>, dim3( nsm - nbig ), dim3( kWg ), 0, s, B.jobs_d.p + nbig, K,
                                 opt.aca_factor * opt.eps, B.rank_d.p + nbig, calls.p );
+         if ( nj > nsm )
+            hipLaunchKernelGGL( k_hm_aca<64>, dim3( nj - nsm ), dim3( 64 ), 0, s, B.jobs_d.p + nsm, K, opt.aca_factor * opt.eps,
+                                B.rank_d.p + nsm, calls.p );
          if ( nbig > 0 && s2 != s ) {
             BH_HIP( hipEventRecord( ev, s2 ) );
             BH_HIP( hipStreamWaitEvent( s, ev, 0 ) );
