@@ -1730,13 +1730,20 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
          B.ttask_d.ensure( std::max<size_t>( tasks_max, 1 ) );
          if ( opt.trunc ) B.tr_ws.ensure( scratch_max );
       }
-      for ( auto& idx : cuts ) {
-         B.idx = std::move( idx );
+      // The large blocks are finished once the last batch's approximations are queued: their
+      // recompression then runs beside those approximations instead of after them.
+      bool big_done = big.idx.empty();
+      for ( size_t c = 0; c < cuts.size(); ++c ) {
+         B.idx = std::move( cuts[ c ] );
          launch( B, st, st_aux );
+         if ( c + 1 == cuts.size() && !big_done ) {
+            finish( big, st_big, st_aux );
+            big_done = true;
+         }
          finish( B, st, st_aux );
       }
       if ( !big.idx.empty() ) {
-         finish( big, st_big, st_aux );
+         if ( !big_done ) finish( big, st_big, st_aux );
          BH_HIP( hipEventRecord( ev, st_big ) );
          BH_HIP( hipStreamWaitEvent( st, ev, 0 ) );
       }
