@@ -321,7 +321,9 @@ __global__ void __launch_bounds__( 256 ) k_transpose( int nt, T* __restrict__ a,
 // dG(a, b)/dtheta; the same with the sign flipped for the collocation point; and M_o : G / area.
 // gsrc: per (collocation block, n2) the 30 sums over the block's lanes (o-major: 9 coordinates,
 // then the area); gcol: per (source strip, n1) 9 values (o-major, 3 coordinates).
-__global__ void __launch_bounds__( 256 ) k_adj_pairs( int n, int strip, double cg, const double* __restrict__ src,
+// Two waves per SIMD although the registers then spill (93 VGPRs): the second wave hides the
+// latency of the pair integrals' transcendental chains, 133 -> 117 ms at 5,120 triangles.
+__global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( 2 ) ) ) k_adj_pairs( int n, int strip, double cg, const double* __restrict__ src,
                                                      const double* __restrict__ xc, const double* __restrict__ lam,
                                                      const double* __restrict__ F, long ld, double* __restrict__ gsrc,
                                                      double* __restrict__ gcol ) {
@@ -355,40 +357,53 @@ __global__ void __launch_bounds__( 256 ) k_adj_pairs( int n, int strip, double c
          wsym[ o ][ 4 ] = M[ 2 ][ 1 ] + M[ 1 ][ 2 ];
          wsym[ o ][ 5 ] = M[ 2 ][ 2 ];
       }
-      // the pair is singular on the diagonal (and meaningless on idle lanes): skipped there
-      double acc[ kAdjOutputs ][ 10 ] = {};
-      if ( live ) {
-         const double ar = s[ kArea ];
-         double g6[ 6 ];
-         pair_block( s, c, cg, g6 );
+      // The pair is singular on the diagonal (and meaningless on idle lanes): its terms are 0 there.
+      // Each term is summed over the wave as soon as it exists (into red, the same fixed order as
+      // a sum after the loop), so no per-direction array stays live across the nine directions.
+      auto wave_store = [&]( int q, const double ( &t )[ kAdjOutputs ] ) {
          for ( int o = 0; o < kAdjOutputs; ++o ) {
-            double t = 0.0;
-            for ( int e = 0; e < 6; ++e ) t += wsym[ o ][ e ] * g6[ e ];
-            acc[ o ][ 9 ] = t / ar;
-         }
-         for ( int dir = 0; dir < 9; ++dir ) {
-            D1 p[ 3 ][ 3 ];
-            for ( int v = 0; v < 3; ++v )
-               for ( int k = 0; k < 3; ++k ) p[ v ][ k ] = mk( s[ kP1 + 3 * v + k ], dir == 3 * v + k ? 1.0 : 0.0 );
-            D1 g[ 6 ];
-            pair_block_t<D1>( p[ 0 ], p[ 1 ], p[ 2 ], c, ar, cg, g );
-            for ( int o = 0; o < kAdjOutputs; ++o ) {
-               double t = 0.0;
-               for ( int e = 0; e < 6; ++e ) t += wsym[ o ][ e ] * g[ e ].d;
-               acc[ o ][ dir ] = t;
-               gc[ o ][ dir % 3 ] -= t;
-            }
-         }
-      }
-      // sums over the workgroup's lanes for source n2, fixed order
-      for ( int o = 0; o < kAdjOutputs; ++o )
-         for ( int q = 0; q < 10; ++q ) {
-            double v = acc[ o ][ q ];
+            double v = t[ o ];
 #pragma unroll
             for ( int off = 32; off > 0; off >>= 1 ) v += __shfl_xor( v, off );
             v = __shfl( v, 0 );
             if ( lane == 0 ) red[ wave ][ o * 10 + q ] = v;
          }
+      };
+      const double ar = s[ kArea ];
+      {
+         double t[ kAdjOutputs ] = {};
+         if ( live ) {
+            double g6[ 6 ];
+            pair_block( s, c, cg, g6 );
+            for ( int o = 0; o < kAdjOutputs; ++o ) {
+               double u = 0.0;
+               for ( int e = 0; e < 6; ++e ) u += wsym[ o ][ e ] * g6[ e ];
+               t[ o ] = u / ar;
+            }
+         }
+         wave_store( 9, t );
+      }
+      for ( int dir = 0; dir < 9; ++dir ) {
+         double t[ kAdjOutputs ] = {};
+         if ( live ) {
+            D1 p[ 3 ][ 3 ];
+            for ( int v = 0; v < 3; ++v )
+               for ( int k = 0; k < 3; ++k ) p[ v ][ k ] = mk( s[ kP1 + 3 * v + k ], dir == 3 * v + k ? 1.0 : 0.0 );
+            D1 g[ 6 ];
+            pair_block_t<D1>( p[ 0 ], p[ 1 ], p[ 2 ], c, ar, cg, g );
+            const int k3 = dir % 3;
+            for ( int o = 0; o < kAdjOutputs; ++o ) {
+               double u = 0.0;
+               for ( int e = 0; e < 6; ++e ) u += wsym[ o ][ e ] * g[ e ].d;
+               t[ o ] = u;
+               // the collocation point moves opposite to the source: selects, not a lane-varying index
+               gc[ o ][ 0 ] -= k3 == 0 ? u : 0.0;
+               gc[ o ][ 1 ] -= k3 == 1 ? u : 0.0;
+               gc[ o ][ 2 ] -= k3 == 2 ? u : 0.0;
+            }
+         }
+         wave_store( dir, t );
+      }
       __syncthreads();
       if ( threadIdx.x < kAdjOutputs * 10 )
          gsrc[ ( (long)blockIdx.x * n + n2 ) * ( kAdjOutputs * 10 ) + threadIdx.x ] =
