@@ -291,6 +291,23 @@ __device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[
    return big;
 }
 
+// v, made opaque to the optimizer ("v": a vector register, "s": scalar registers).  The leaf
+// routines below are inlined into the diagonal block's loop over its four column blocks; from
+// there the compiler would hoist the 32 per-step lane masks (i > j) and the 32 column offsets of
+// the leaf's global stores out of the loop, and they outgrow the scalar registers: spilled to
+// vector-register lanes, every use costs readlanes and hazard waits.  An opaque row index (and
+// leading dimension) keeps each mask a single compare where it is used.
+template <typename V>
+__device__ __forceinline__ V opaque_v( V v ) {
+   asm volatile( "" : "+v"( v ) );
+   return v;
+}
+template <typename V>
+__device__ __forceinline__ V opaque_s( V v ) {
+   asm volatile( "" : "+s"( v ) );
+   return v;
+}
+
 // One wave factors the 32 x 32 tile held as rows in registers (lane i and i + 32: row i, a[c] =
 // entry (i, c)) without exchanges: multipliers a_ij * (1 / a_jj), as DGETF2 scales; pivot-row
 // entries come from the pivot lane by readlane, so the 32 steps need no barrier.  Returns true
@@ -298,6 +315,7 @@ __device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[
 // would exchange rows).
 template <typename T>
 __device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i, T thr ) {
+   i = opaque_v( i );
    bool big = false;
 #pragma unroll
    for ( int j = 0; j < 32; ++j ) {
@@ -322,6 +340,7 @@ __device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i, T thr ) {
 // is untouched): for k = 0..31, rows i > k get X(i, c) -= l_ik X(k, c) for c < k and X(i, k) = -l_ik.
 template <typename T>
 __device__ __forceinline__ void leaf_inv_lower( T ( &a )[ 32 ], int i ) {
+   i = opaque_v( i );
 #pragma unroll
    for ( int k = 0; k < 32; ++k ) {
       const T li = i > k ? a[ k ] : T( 0 );
@@ -342,6 +361,7 @@ __device__ __forceinline__ void leaf_inv_lower( T ( &a )[ 32 ], int i ) {
 // lane's row index would make the compiler send a[] through scratch memory.
 template <typename T>
 __device__ __forceinline__ void leaf_inv_upper( T ( &a )[ 32 ], int i, T dg ) {
+   i = opaque_v( i );
    const T rd = T( 1 ) / dg;
 #pragma unroll
    for ( int c = 1; c < 32; ++c ) a[ c ] = c > i ? a[ c ] * rd : a[ c ];
@@ -376,6 +396,7 @@ __device__ __forceinline__ constexpr bool leaf_owns( int c ) {
 
 template <int P, typename T>
 __device__ __forceinline__ void leaf_inv_lower_part( T ( &a )[ 32 ], int i ) {
+   i = opaque_v( i );
 #pragma unroll
    for ( int k = 0; k < 32; ++k ) {
       const T li = i > k ? a[ k ] : T( 0 );
@@ -392,6 +413,7 @@ __device__ __forceinline__ void leaf_inv_lower_part( T ( &a )[ 32 ], int i ) {
 
 template <int P, typename T>
 __device__ __forceinline__ void leaf_inv_upper_part( T ( &a )[ 32 ], int i, T dg ) {
+   i = opaque_v( i );
    const T rd = T( 1 ) / dg;
    // every column is scaled: the multipliers u1_ik of the other part's columns are needed too
 #pragma unroll
@@ -461,17 +483,20 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
    T a[ 32 ];
    if ( mode == 0 ) {
       for ( int p = 0; p < 4; ++p ) {
+         // the lane and its row, opaque per block (see opaque_v)
+         const int ln = opaque_v( lane ), ri = ln & 31;
          T* Spp = tile( p, p );
          if ( wave == 0 ) {
 #pragma unroll
-            for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
-            big |= leaf_lu( a, i, thr );
-            if ( lane < 32 ) {
-               T* Ap = A + (long)( 32 * p ) * lda + 32 * p;
+            for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + ri ];
+            big |= leaf_lu( a, ri, thr );
+            if ( ln < 32 ) {
+               const long ld = opaque_s( lda );
+               T* Ap = A + (long)( 32 * p ) * ld + 32 * p;
 #pragma unroll
                for ( int c = 0; c < 32; ++c ) {
-                  Spp[ c * kLd + i ] = a[ c ];
-                  Ap[ (long)c * lda + i ] = a[ c ];
+                  Spp[ c * kLd + ri ] = a[ c ];
+                  Ap[ (long)c * ld + ri ] = a[ c ];
                }
             }
          }
@@ -480,32 +505,32 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
          // the factored tile's rows first (wave 0 still holds them), then all write their columns
          if ( wave == 1 ) {
 #pragma unroll
-            for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + i ];
+            for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + ri ];
          } else if ( wave >= 2 ) {
 #pragma unroll
-            for ( int c = 0; c < 32; ++c ) a[ c ] = c >= i ? Spp[ c * kLd + i ] : T( 0 );
+            for ( int c = 0; c < 32; ++c ) a[ c ] = c >= ri ? Spp[ c * kLd + ri ] : T( 0 );
          }
-         const T dg = Spp[ i * kLd + i ];
+         const T dg = Spp[ ri * kLd + ri ];
          __syncthreads();
          if ( wave < 2 ) {
             if ( wave == 0 )
-               leaf_inv_lower_part<0>( a, i );
+               leaf_inv_lower_part<0>( a, ri );
             else
-               leaf_inv_lower_part<1>( a, i );
-            if ( lane < 32 ) {
+               leaf_inv_lower_part<1>( a, ri );
+            if ( ln < 32 ) {
 #pragma unroll
                for ( int c = 0; c < 31; ++c )
-                  if ( c < i && ( wave == 0 ? leaf_owns<0>( c ) : leaf_owns<1>( c ) ) ) Spp[ c * kLd + i ] = a[ c ];
+                  if ( c < ri && ( wave == 0 ? leaf_owns<0>( c ) : leaf_owns<1>( c ) ) ) Spp[ c * kLd + ri ] = a[ c ];
             }
          } else {
             if ( wave == 2 )
-               leaf_inv_upper_part<0>( a, i, dg );
+               leaf_inv_upper_part<0>( a, ri, dg );
             else
-               leaf_inv_upper_part<1>( a, i, dg );
-            if ( lane < 32 ) {
+               leaf_inv_upper_part<1>( a, ri, dg );
+            if ( ln < 32 ) {
 #pragma unroll
                for ( int c = 0; c < 32; ++c )
-                  if ( c >= i && ( wave == 2 ? leaf_owns<0>( c ) : leaf_owns<1>( c ) ) ) Spp[ c * kLd + i ] = a[ c ];
+                  if ( c >= ri && ( wave == 2 ? leaf_owns<0>( c ) : leaf_owns<1>( c ) ) ) Spp[ c * kLd + ri ] = a[ c ];
             }
          }
          __syncthreads();
@@ -515,21 +540,21 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
             tile_zero<T>( acc );
             if ( t < nr ) {  // L panel: A U_pp^-1 (the multipliers: checked against 1)
                T* C = tile( p + 1 + t, p );
-               tile_mma<T, kOpFull, kOpUpper>( acc, C, Spp, T( 1 ), lane );
-               big |= tile_store<T>( acc, C, lane, true, thr );
+               tile_mma<T, kOpFull, kOpUpper>( acc, C, Spp, T( 1 ), ln );
+               big |= tile_store<T>( acc, C, ln, true, thr );
             } else {  // U row: L_pp^-1 A
                T* C = tile( p, p + 1 + t - nr );
-               tile_mma<T, kOpLowerUnit, kOpFull>( acc, Spp, C, T( 1 ), lane );
-               tile_store<T>( acc, C, lane, false );
+               tile_mma<T, kOpLowerUnit, kOpFull>( acc, Spp, C, T( 1 ), ln );
+               tile_store<T>( acc, C, ln, false );
             }
          }
          __syncthreads();
          for ( int t = wave; t < nr * nr; t += 4 ) {
             const int I = p + 1 + t % nr, J = p + 1 + t / nr;
             v4 acc[ 2 ][ 2 ];
-            tile_load<T>( acc, tile( I, J ), lane );
-            tile_mma<T, kOpFull, kOpFull>( acc, tile( I, p ), tile( p, J ), T( -1 ), lane );
-            tile_store<T>( acc, tile( I, J ), lane, false );
+            tile_load<T>( acc, tile( I, J ), ln );
+            tile_mma<T, kOpFull, kOpFull>( acc, tile( I, p ), tile( p, J ), T( -1 ), ln );
+            tile_store<T>( acc, tile( I, J ), ln, false );
          }
          __syncthreads();
       }
