@@ -291,21 +291,31 @@ __device__ __forceinline__ bool tile_store( const typename Mfma<T>::v4 ( &acc )[
    return big;
 }
 
-// v, made opaque to the optimizer ("v": a vector register, "s": scalar registers).  The leaf
-// routines below are inlined into the diagonal block's loop over its four column blocks; from
-// there the compiler would hoist the 32 per-step lane masks (i > j) and the 32 column offsets of
-// the leaf's global stores out of the loop, and they outgrow the scalar registers: spilled to
-// vector-register lanes, every use costs readlanes and hazard waits.  An opaque row index (and
-// leading dimension) keeps each mask a single compare where it is used.
+// v, made opaque to the optimizer (held in a vector register).  The leaf routines below are
+// inlined into the diagonal block's loop over its four column blocks; from there the compiler
+// would hoist the per-step lane masks (i > j, and the triangle masks of the tile products) out of
+// the loop, and they outgrow the scalar registers: spilled to vector-register lanes, every use
+// costs readlanes and hazard waits.  An opaque lane index keeps each mask a single compare where
+// it is used.
 template <typename V>
 __device__ __forceinline__ V opaque_v( V v ) {
    asm volatile( "" : "+v"( v ) );
    return v;
 }
-template <typename V>
-__device__ __forceinline__ V opaque_s( V v ) {
-   asm volatile( "" : "+s"( v ) );
-   return v;
+
+// 1 / v for the leaf factorization's multipliers: FP32 takes the hardware reciprocal (v_rcp_f32,
+// within 1 ulp; the FP32 factors only seed the FP64 refinement, and the correctly rounded quotient
+// costs ten dependent instructions per column); FP64 the correctly rounded quotient, as DGETF2
+// scales by ONE / A(J,J).
+template <typename T>
+__device__ inline T leaf_recip( T v );
+template <>
+__device__ inline float leaf_recip<float>( float v ) {
+   return __builtin_amdgcn_rcpf( v );
+}
+template <>
+__device__ inline double leaf_recip<double>( double v ) {
+   return 1.0 / v;
 }
 
 // One wave factors the 32 x 32 tile held as rows in registers (lane i and i + 32: row i, a[c] =
@@ -320,7 +330,7 @@ __device__ __forceinline__ bool leaf_lu( T ( &a )[ 32 ], int i, T thr ) {
 #pragma unroll
    for ( int j = 0; j < 32; ++j ) {
       const T piv = bcast( a[ j ], j );
-      const T rp = T( 1 ) / piv;
+      const T rp = leaf_recip( piv );
       const bool below = i > j;
       big |= below && !( tabs( a[ j ] ) <= thr * tabs( piv ) );
       const T l = below ? a[ j ] * rp : T( 0 );
@@ -491,16 +501,26 @@ __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T*
             for ( int c = 0; c < 32; ++c ) a[ c ] = Spp[ c * kLd + ri ];
             big |= leaf_lu( a, ri, thr );
             if ( ln < 32 ) {
-               const long ld = opaque_s( lda );
-               T* Ap = A + (long)( 32 * p ) * ld + 32 * p;
 #pragma unroll
-               for ( int c = 0; c < 32; ++c ) {
-                  Spp[ c * kLd + ri ] = a[ c ];
-                  Ap[ (long)c * ld + ri ] = a[ c ];
-               }
+               for ( int c = 0; c < 32; ++c ) Spp[ c * kLd + ri ] = a[ c ];
             }
          }
          __syncthreads();
+         {
+            // the factored tile to A before its inverses overwrite it: 16-byte stores over the
+            // whole workgroup, beside the row reads below
+            constexpr int V = 16 / sizeof( T ), NV = 32 * 32 / V / 256;
+            typedef T vT __attribute__( ( ext_vector_type( V ) ) );
+            T* Ap = A + (long)( 32 * p ) * lda + 32 * p;
+#pragma unroll
+            for ( int q = 0; q < NV; ++q ) {
+               const int v = q * 256 + tid, c = v / ( 32 / V ), r = ( v % ( 32 / V ) ) * V;
+               vT x;
+#pragma unroll
+               for ( int u = 0; u < V; ++u ) x[ u ] = Spp[ c * kLd + r + u ];
+               *reinterpret_cast<vT*>( Ap + (long)c * lda + r ) = x;
+            }
+         }
          // L^-1 on waves 0 and 1, U^-1 on waves 2 and 3, half of the columns each: every wave reads
          // the factored tile's rows first (wave 0 still holds them), then all write their columns
          if ( wave == 1 ) {
