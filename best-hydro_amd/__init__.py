@@ -265,14 +265,16 @@ class Solver:
         self.L.bhg_get_stats(self.h, C.byref(s))
         return s.as_dict()
 
-    PROFILE_CLASSES = ("pairs", "gemm64", "gemm32", "diag", "trsm", "resid", "other")
+    # gemm32: FP32 GEMM launches on tiles below 128 x 128; gemm32_128: on 128 x 128 tiles
+    PROFILE_CLASSES = ("pairs", "gemm64", "gemm32", "diag", "trsm", "resid", "other", "gemm32_128")
 
     def set_profiling(self, on=True):
         self.L.bhg_set_profiling(self.h, int(bool(on)))
 
     def profile(self):
-        ms, fl = np.zeros(7), np.zeros(7)
-        cnt = np.zeros(7, dtype=np.int64)
+        n = len(self.PROFILE_CLASSES)
+        ms, fl = np.zeros(n), np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
         self.L.bhg_get_profile(self.h, ms, fl, cnt)
         return {k: dict(ms=float(ms[i]), flops=float(fl[i]), launches=int(cnt[i]))
                 for i, k in enumerate(self.PROFILE_CLASSES)}

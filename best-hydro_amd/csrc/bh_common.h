@@ -43,15 +43,18 @@ enum SrcField {
 
 // The streams and events of the blocked factorization: st (the caller's; the critical path),
 // st2 (the trailing update), st3 (the U row panels); ev_u / ev_g order st2 after st and back, ev
-// the forks and joins of st3.
+// the forks and joins of st3 (ev[0..4]) and the later parts of st2's update (ev[5], ev[6]).
+constexpr int kLuEvents = 7;
 struct LuStreams {
    hipStream_t st, st2, st3;
    hipEvent_t ev_u, ev_g;
-   hipEvent_t ev[ 5 ];
+   hipEvent_t ev[ kLuEvents ];
 };
 
 // Kernel classes timed by the optional launch profiler.
-enum KernelClass { kKPairs = 0, kKGemm64, kKGemm32, kKDiag, kKTrsm, kKResid, kKOther, kKClasses };
+// kKGemm32 counts the FP32 GEMMs on tiles smaller than 128 x 128 (the panels, row panels and the
+// narrow parts of the trailing update), kKGemm32T those on 128 x 128 tiles (the wide trailing updates).
+enum KernelClass { kKPairs = 0, kKGemm64, kKGemm32, kKDiag, kKTrsm, kKResid, kKOther, kKGemm32T, kKClasses };
 
 // When a solver turns profiling on, every launch of a timed class is bracketed by two HIP events
 // on the launching stream; the algorithmic flops of the launch are added to its class.  The

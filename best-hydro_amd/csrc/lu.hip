@@ -1039,14 +1039,15 @@ void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T*
                           T* C, long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb,
                           BatchStride sc ) {
    if ( M <= 0 || N <= 0 || batch <= 0 ) return;
-   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32, 2.0 * M * N * K * batch, st );
    const long tiles = (long)( M / 128 ) * ( N / 128 ) * batch;
    constexpr long kSmall = 256;  // CUs
-   if ( tiles < kSmall && C == A && N == 128 )
+   const int shape = tiles >= kSmall ? 0 : C == A && N == 128 ? 1 : C == B && M == 128 ? 2 : C != A && C != B ? 3 : 0;
+   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : shape == 0 ? kKGemm32T : kKGemm32, 2.0 * M * N * K * batch, st );
+   if ( shape == 1 )
       launch_gemm_shape<T, 64, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
-   else if ( tiles < kSmall && C == B && M == 128 )
+   else if ( shape == 2 )
       launch_gemm_shape<T, 128, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
-   else if ( tiles < kSmall && C != A && C != B )
+   else if ( shape == 3 )
       launch_gemm_shape<T, 64, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
    else
       launch_gemm_shape<T, 128, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
@@ -1068,11 +1069,14 @@ template void launch_gemm<float>( hipStream_t, int, int, int, float, const float
 // column by the first), then the two U row panels are formed and the trailing matrix gets one
 // rank-256 update, which halves the passes over the trailing matrix of a rank-128 schedule.  The
 // trailing update is split: the next pair of block columns on st, so the next pair's diagonal
-// blocks start at once, and the rest on st2, overlapping them.  The U row panels are off the
-// critical path on st3: U(k0, k1) beside the panel below k0, the row panels right of the pair
-// (which wait for the previous pair's update on st2, whose rows they read and write) beside the
-// second block column's factorization, so st runs only diagonal block, panel, update, diagonal
-// block, panel, next-columns update.  check is set when partial pivoting would have exchanged
+// blocks start at once, and the rest on st2, overlapping them.  st2 takes the rest in three
+// launches: first the next pair's two block rows (A), then the two block columns after the next
+// pair (N), then the remainder (B); the next pair waits for A and N only, so its whole
+// factorization runs beside B and st2 goes on to the next update without a gap.  The U row panels
+// are off the critical path on st3: U(k0, k1) beside the panel below k0, the row panels right of
+// the pair (which wait for part A of the previous pair's update, the rows they read and write)
+// beside the second block column's factorization, so st runs only diagonal block, panel, update,
+// diagonal block, panel, next-columns update.  check is set when partial pivoting would have exchanged
 // rows somewhere; the caller must then rerun lu_pivoted on the original matrix.
 template <typename T>
 void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check ) {
@@ -1096,7 +1100,9 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
       BH_HIP( hipEventRecord( e, from ) );
       BH_HIP( hipStreamWaitEvent( to, e, 0 ) );
    };
-   bool pending = false;  // an update on st2 that the next pair's U row panels must wait for
+   bool pending = false;    // an update on st2 (part A) that the next pair's U row panels must wait for
+   bool pending_n = false;  // an update on st2 (part N) that the next pair's next-columns update must wait for
+   bool used2 = false;      // st2 has work to join at the end
    for ( int k0 = 0; k0 < nblk; k0 += 2 ) {
       const int k1 = k0 + 1;
       if ( k1 >= nblk ) {  // a last single block (npad an odd multiple of 128)
@@ -1133,18 +1139,36 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
       BH_HIP( hipStreamWaitEvent( st, S.ev[ 4 ], 0 ) );
       // the rank-256 trailing update: L = the two panels below, U = the two row panels
       const int nnext = std::min( 2 * nb, rest );
+      const bool more = rest > nnext;
+      if ( more ) order( S.ev_u, st, st2 );  // st2 reads this pair's panels, not the next columns
+      // the next pair's columns, after the previous update's part N has reached them
+      if ( pending_n ) BH_HIP( hipStreamWaitEvent( st, S.ev[ 5 ], 0 ) );
+      pending_n = false;
       launch_gemm<T>( st, rest, nnext, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( c2, c2 ), lda,
                       nullptr );
-      if ( rest > nnext ) {
-         order( S.ev_u, st, st2 );
-         launch_gemm<T>( st2, rest, rest - nnext, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, c2 + 2 ), lda, T( 1 ),
-                         blk( c2, c2 + 2 ), lda, nullptr );
+      if ( more ) {
+         // The rest on st2, in three parts so that the next pair waits only for what it reads:
+         // A, the next pair's rows (its U row panels, on st3); N, the columns of the pair after it
+         // (its next-columns update, on st); B, the remainder, which overlaps the next pair.
+         const int cr = c2 + nnext / nb, wr = rest - nnext, nn = std::min( 2 * nb, wr );
+         launch_gemm<T>( st2, nnext, wr, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( c2, cr ), lda,
+                         nullptr );
          BH_HIP( hipEventRecord( S.ev_g, st2 ) );
          pending = true;
+         launch_gemm<T>( st2, wr, nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( cr, cr ), lda,
+                         nullptr );
+         BH_HIP( hipEventRecord( S.ev[ 5 ], st2 ) );
+         pending_n = true;
+         if ( wr > nn )
+            launch_gemm<T>( st2, wr, wr - nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr + nn / nb ), lda, T( 1 ),
+                            blk( cr, cr + nn / nb ), lda, nullptr );
+         BH_HIP( hipEventRecord( S.ev[ 6 ], st2 ) );
+         used2 = true;
       }
    }
-   // the caller's later work on st must see st2's last update (st3's work is joined into st above)
-   if ( pending ) BH_HIP( hipStreamWaitEvent( st, S.ev_g, 0 ) );
+   // the caller's later work on st must see all of st2's updates (st3's work is joined into st
+   // above)
+   if ( used2 ) BH_HIP( hipStreamWaitEvent( st, S.ev[ 6 ], 0 ) );
 }
 template void lu_nopiv<double>( const LuStreams&, int, double*, long, double*, int* );
 template void lu_nopiv<float>( const LuStreams&, int, float*, long, float*, int* );

@@ -99,9 +99,11 @@ struct bhg_solver {
    hipStream_t st2 = nullptr;          // look-ahead stream of the factorization
    hipStream_t st3 = nullptr;          // its U row panels
    hipEvent_t ev_u = nullptr, ev_g = nullptr;
-   hipEvent_t ev_lu[ 5 ] = {};
+   hipEvent_t ev_lu[ bhg::kLuEvents ] = {};
    bhg::LuStreams lu_streams() const {
-      return { st, st2, st3, ev_u, ev_g, { ev_lu[ 0 ], ev_lu[ 1 ], ev_lu[ 2 ], ev_lu[ 3 ], ev_lu[ 4 ] } };
+      bhg::LuStreams l { st, st2, st3, ev_u, ev_g, {} };
+      std::copy( ev_lu, ev_lu + bhg::kLuEvents, l.ev );
+      return l;
    }
    std::string last_error;
    std::deque<std::string> log;
@@ -973,8 +975,8 @@ void bhg_destroy( bhg_solver* s ) {
    if ( s->st3 ) (void)hipStreamSynchronize( s->st3 );
    hipStream_t st = s->st, st2 = s->st2, st3 = s->st3;
    hipEvent_t e1 = s->ev_u, e2 = s->ev_g;
-   hipEvent_t el[ 5 ];
-   std::copy( s->ev_lu, s->ev_lu + 5, el );
+   hipEvent_t el[ bhg::kLuEvents ];
+   std::copy( s->ev_lu, s->ev_lu + bhg::kLuEvents, el );
    delete s;
    if ( e1 ) (void)hipEventDestroy( e1 );
    if ( e2 ) (void)hipEventDestroy( e2 );
