@@ -53,106 +53,129 @@ double now() {
 }
 
 // ------------------------------------------------------------------ cluster and block trees (host)
-
-struct Cluster {
-   int beg = 0, n = 0, level = 0;
-   int son[ 2 ] = { -1, -1 };
-   double lo[ 3 ] = { 0, 0, 0 }, hi[ 3 ] = { 0, 0, 0 };  // box of the triangles' vertices
-   int leaf_first = 0, leaf_count = 0;
-   bool leaf() const { return son[ 0 ] < 0; }
-};
+//
+// The rules are the reference's (hmatrix/src/cluster.cpp:13-117, hmatrix/src/hmatrix.cpp:27-54):
+// a cluster is halved at the median incenter along the longest side of its incenters' box (the
+// first half the floor(n/2) smallest by (coordinate, triangle)); every cluster of a level is
+// halved, down to the first depth at which no cluster exceeds `leaf` triangles; a pair of clusters
+// is admissible when the smaller diagonal of their vertex boxes is at most eta times the gap
+// between the boxes.  The layout is this file's own: since every level is split whole, the tree
+// is a complete binary tree stored in heap order (level l is clusters 2^l - 1 .. 2^(l+1) - 2, left
+// to right; the sons of c are 2c + 1 and 2c + 2; the leaves are the last 2^depth, in tree order),
+// one array per field, built one level at a time, and the block tree is enumerated level by level.
 
 struct Tree {
-   std::vector<Cluster> c;
-   std::vector<int> perm, pos, leaves;  // tree position -> triangle, triangle -> position, leaf clusters
-   int depth = 0;
+   int n = 0, depth = 0;
+   std::vector<int> beg, cnt;     // cluster c holds the triangles perm[beg[c] .. beg[c] + cnt[c])
+   std::vector<double> box;       // cluster c: box[6c .. 6c + 2] lowest, [6c + 3 .. 6c + 5] highest vertex coordinates
+   std::vector<int> perm, pos;    // tree position -> triangle, triangle -> tree position
+   int clusters() const { return ( 2 << depth ) - 1; }
+   int nleaves() const { return 1 << depth; }
+   int first_leaf() const { return ( 1 << depth ) - 1; }
+   bool is_leaf( int c ) const { return c >= first_leaf(); }
+   static int level_of( int c ) {
+      int l = 0;
+      while ( ( 2 << l ) - 1 <= c ) ++l;
+      return l;
+   }
+   // the leaves below c, as leaf indices lo .. lo + count - 1
+   void leaf_range( int c, int& lo, int& count ) const {
+      const int l = level_of( c );
+      count = 1 << ( depth - l );
+      lo = ( c - ( ( 1 << l ) - 1 ) ) * count;
+   }
+   int leaf_cluster( int i ) const { return first_leaf() + i; }
 };
 
 // v: v1, v2, v3 as n x 3 column-major arrays one after the other; x: incenters n x 3 column-major.
-void split( Tree& T, int n, const double* v, const double* x, int id ) {
-   Cluster cl = T.c[ id ];
-   for ( int d = 0; d < 3; ++d ) cl.lo[ d ] = 1e300, cl.hi[ d ] = -1e300;
-   for ( int k = cl.beg; k < cl.beg + cl.n; ++k ) {
-      const int t = T.perm[ k ];
-      for ( int j = 0; j < 3; ++j )
-         for ( int d = 0; d < 3; ++d ) {
-            const double w = v[ (size_t)j * 3 * n + (size_t)d * n + t ];
-            cl.lo[ d ] = std::min( cl.lo[ d ], w );
-            cl.hi[ d ] = std::max( cl.hi[ d ], w );
-         }
-   }
-   if ( cl.level == T.depth ) {
-      cl.leaf_first = (int)T.leaves.size();
-      cl.leaf_count = 1;
-      T.leaves.push_back( id );
-      T.c[ id ] = cl;
-      return;
-   }
-   double lo[ 3 ] = { 1e300, 1e300, 1e300 }, hi[ 3 ] = { -1e300, -1e300, -1e300 };
-   for ( int k = cl.beg; k < cl.beg + cl.n; ++k ) {
-      const int t = T.perm[ k ];
-      for ( int d = 0; d < 3; ++d ) {
-         lo[ d ] = std::min( lo[ d ], x[ (size_t)d * n + t ] );
-         hi[ d ] = std::max( hi[ d ], x[ (size_t)d * n + t ] );
-      }
-   }
-   int axis = 0;
-   for ( int d = 1; d < 3; ++d )
-      if ( hi[ d ] - lo[ d ] > hi[ axis ] - lo[ axis ] ) axis = d;
-   // The first half are the n/2 smallest by (coordinate, triangle): the same split as a selection
-   // with that order; sorting also fixes the order inside each half.
-   int* b = T.perm.data() + cl.beg;
-   std::sort( b, b + cl.n, [&]( int p, int q ) {
-      const double xp = x[ (size_t)axis * n + p ], xq = x[ (size_t)axis * n + q ];
-      return xp < xq || ( xp == xq && p < q );
-   } );
-   const int half = cl.n / 2;
-   Cluster s0, s1;
-   s0.beg = cl.beg, s0.n = half;
-   s1.beg = cl.beg + half, s1.n = cl.n - half;
-   s0.level = s1.level = cl.level + 1;
-   const int i0 = (int)T.c.size();
-   T.c.push_back( s0 );
-   T.c.push_back( s1 );
-   cl.son[ 0 ] = i0, cl.son[ 1 ] = i0 + 1;
-   cl.leaf_first = (int)T.leaves.size();
-   T.c[ id ] = cl;
-   split( T, n, v, x, i0 );
-   split( T, n, v, x, i0 + 1 );
-   T.c[ id ].leaf_count = (int)T.leaves.size() - T.c[ id ].leaf_first;
-}
-
 Tree build_tree( int n, const double* v, const double* x, int leaf ) {
    Tree T;
+   T.n = n;
+   while ( n > 0 && ( ( n - 1 ) >> T.depth ) + 1 > leaf ) ++T.depth;
+   const int nc = T.clusters();
+   T.beg.assign( nc, 0 );
+   T.cnt.assign( nc, 0 );
+   T.box.assign( 6 * (size_t)nc, 0.0 );
    T.perm.resize( n );
    std::iota( T.perm.begin(), T.perm.end(), 0 );
-   while ( ( ( (long)n + ( 1L << T.depth ) - 1 ) >> T.depth ) > leaf ) ++T.depth;
-   Cluster root;
-   root.n = n;
-   T.c.reserve( (size_t)2 << T.depth );
-   T.c.push_back( root );
-   split( T, n, v, x, 0 );
+   T.cnt[ 0 ] = n;
+   // one level at a time: each cluster's split axis, then one sort of all positions by (cluster,
+   // coordinate on its axis, triangle), which orders every cluster's segment in place
+   std::vector<int> owner( n, 0 ), axis( 1 << std::max( T.depth - 1, 0 ), 0 );
+   for ( int l = 0; l < T.depth; ++l ) {
+      const int c0 = ( 1 << l ) - 1, nl = 1 << l;
+      for ( int k = 0; k < nl; ++k ) {
+         const int c = c0 + k;
+         double lo[ 3 ] = { HUGE_VAL, HUGE_VAL, HUGE_VAL }, hi[ 3 ] = { -HUGE_VAL, -HUGE_VAL, -HUGE_VAL };
+         for ( int q = T.beg[ c ]; q < T.beg[ c ] + T.cnt[ c ]; ++q ) {
+            owner[ q ] = k;
+            for ( int d = 0; d < 3; ++d ) {
+               const double w = x[ (size_t)d * n + T.perm[ q ] ];
+               lo[ d ] = std::min( lo[ d ], w );
+               hi[ d ] = std::max( hi[ d ], w );
+            }
+         }
+         int a = 0;
+         for ( int d = 1; d < 3; ++d )
+            if ( hi[ d ] - lo[ d ] > hi[ a ] - lo[ a ] ) a = d;
+         axis[ k ] = a;
+      }
+      std::vector<int> order( n );
+      std::iota( order.begin(), order.end(), 0 );
+      std::sort( order.begin(), order.end(), [&]( int p, int q ) {
+         if ( owner[ p ] != owner[ q ] ) return owner[ p ] < owner[ q ];
+         const int a = axis[ owner[ p ] ], tp = T.perm[ p ], tq = T.perm[ q ];
+         const double xp = x[ (size_t)a * n + tp ], xq = x[ (size_t)a * n + tq ];
+         return xp != xq ? xp < xq : tp < tq;
+      } );
+      std::vector<int> next( n );
+      for ( int q = 0; q < n; ++q ) next[ q ] = T.perm[ order[ q ] ];
+      T.perm.swap( next );
+      for ( int k = 0; k < nl; ++k ) {
+         const int c = c0 + k, s0 = 2 * c + 1, s1 = 2 * c + 2;
+         T.beg[ s0 ] = T.beg[ c ];
+         T.cnt[ s0 ] = T.cnt[ c ] / 2;
+         T.beg[ s1 ] = T.beg[ c ] + T.cnt[ s0 ];
+         T.cnt[ s1 ] = T.cnt[ c ] - T.cnt[ s0 ];
+      }
+   }
+   // vertex boxes: the leaves from their triangles, every other cluster the union of its sons'
+   for ( int c = nc - 1; c >= 0; --c ) {
+      double* bx = &T.box[ 6 * (size_t)c ];
+      for ( int d = 0; d < 3; ++d ) bx[ d ] = HUGE_VAL, bx[ 3 + d ] = -HUGE_VAL;
+      if ( T.is_leaf( c ) ) {
+         for ( int q = T.beg[ c ]; q < T.beg[ c ] + T.cnt[ c ]; ++q )
+            for ( int j = 0; j < 3; ++j )
+               for ( int d = 0; d < 3; ++d ) {
+                  const double w = v[ (size_t)j * 3 * n + (size_t)d * n + T.perm[ q ] ];
+                  bx[ d ] = std::min( bx[ d ], w );
+                  bx[ 3 + d ] = std::max( bx[ 3 + d ], w );
+               }
+      } else {
+         for ( int i = 1; i <= 2; ++i ) {
+            const double* sb = &T.box[ 6 * (size_t)( 2 * c + i ) ];
+            for ( int d = 0; d < 3; ++d ) bx[ d ] = std::min( bx[ d ], sb[ d ] ), bx[ 3 + d ] = std::max( bx[ 3 + d ], sb[ 3 + d ] );
+         }
+      }
+   }
    T.pos.resize( n );
-   for ( int k = 0; k < n; ++k ) T.pos[ T.perm[ k ] ] = k;
+   for ( int q = 0; q < n; ++q ) T.pos[ T.perm[ q ] ] = q;
    return T;
 }
 
-double diam( const Cluster& t ) {
-   double s = 0;
-   for ( int d = 0; d < 3; ++d ) s += ( t.hi[ d ] - t.lo[ d ] ) * ( t.hi[ d ] - t.lo[ d ] );
-   return std::sqrt( s );
-}
-
-double dist( const Cluster& t, const Cluster& u ) {
-   double s = 0;
+// The smaller box diagonal at most eta times the distance between the boxes.
+bool admissible_pair( const Tree& T, int t, int s, double eta ) {
+   const double* a = &T.box[ 6 * (size_t)t ];
+   const double* b = &T.box[ 6 * (size_t)s ];
+   double da = 0.0, db = 0.0, gap = 0.0;
    for ( int d = 0; d < 3; ++d ) {
-      const double gap = std::max( 0.0, std::max( t.lo[ d ] - u.hi[ d ], u.lo[ d ] - t.hi[ d ] ) );
-      s += gap * gap;
+      da += ( a[ 3 + d ] - a[ d ] ) * ( a[ 3 + d ] - a[ d ] );
+      db += ( b[ 3 + d ] - b[ d ] ) * ( b[ 3 + d ] - b[ d ] );
+      const double g = std::max( 0.0, std::max( a[ d ] - b[ 3 + d ], b[ d ] - a[ 3 + d ] ) );
+      gap += g * g;
    }
-   return std::sqrt( s );
+   return std::sqrt( std::min( da, db ) ) <= eta * std::sqrt( gap );
 }
-
-bool admissible( const Cluster& t, const Cluster& s, double eta ) { return std::min( diam( t ), diam( s ) ) <= eta * dist( t, s ); }
 
 enum BlkType { kDense = 0, kLowRank = 1 };
 
@@ -164,23 +187,31 @@ struct Blk {
    long nr, nc, r0, c0;  // rows and columns (unknowns, tree order)
 };
 
-void make_blocks( const Tree& T, int t, int s, const HmOptions& opt, std::vector<Blk>& out ) {
-   const Cluster& ct = T.c[ t ];
-   const Cluster& cs = T.c[ s ];
-   Blk b { t, s, kDense, false, false, 0, 3L * ct.n, 3L * cs.n, 3L * ct.beg, 3L * cs.beg };
-   if ( t != s && admissible( ct, cs, opt.eta ) ) {
-      b.small = (long)ct.n * cs.n <= opt.dense_small;
-      b.type = b.small ? kDense : kLowRank;
-      out.push_back( b );
-      return;
+// The leaf blocks of the block tree, coarsest level first: a pair of clusters of one level is a
+// leaf when it is admissible (low-rank, or dense when small) or when both are leaf clusters
+// (dense); otherwise its four pairs of sons go to the next level.
+std::vector<Blk> make_blocks( const Tree& T, const HmOptions& opt ) {
+   std::vector<Blk> out;
+   std::vector<std::pair<int, int>> level { { 0, 0 } }, next;
+   while ( !level.empty() ) {
+      next.clear();
+      for ( const auto& ts : level ) {
+         const int t = ts.first, s = ts.second;
+         Blk b { t, s, kDense, false, false, 0, 3L * T.cnt[ t ], 3L * T.cnt[ s ], 3L * T.beg[ t ], 3L * T.beg[ s ] };
+         if ( t != s && admissible_pair( T, t, s, opt.eta ) ) {
+            b.small = (long)T.cnt[ t ] * T.cnt[ s ] <= opt.dense_small;
+            b.type = b.small ? kDense : kLowRank;
+            out.push_back( b );
+         } else if ( T.is_leaf( t ) ) {
+            out.push_back( b );  // t and s are on one level: both leaves
+         } else {
+            for ( int i = 1; i <= 2; ++i )
+               for ( int j = 1; j <= 2; ++j ) next.emplace_back( 2 * t + i, 2 * s + j );
+         }
+      }
+      level.swap( next );
    }
-   if ( ct.leaf() || cs.leaf() ) {
-      if ( !( ct.leaf() && cs.leaf() ) ) throw std::runtime_error( "cluster tree is not level-consistent" );
-      out.push_back( b );
-      return;
-   }
-   for ( int i = 0; i < 2; ++i )
-      for ( int j = 0; j < 2; ++j ) make_blocks( T, ct.son[ i ], cs.son[ j ], opt, out );
+   return out;
 }
 
 // ------------------------------------------------------------------ device helpers
@@ -1368,12 +1399,14 @@ void HMatrixDeleter::operator()( HMatrix* h ) const { delete h; }
 HmStats hm_plan( int n, const double* h_v, const double* h_xc, const HmOptions& opt ) {
    HmStats S;
    Tree T = build_tree( n, h_v, h_xc, opt.leaf );
-   std::vector<Blk> blks;
-   make_blocks( T, 0, 0, opt, blks );
+   const std::vector<Blk> blks = make_blocks( T, opt );
    S.depth = T.depth;
-   S.nleaves = (int)T.leaves.size();
+   S.nleaves = T.nleaves();
    S.leaf_min = 1 << 30;
-   for ( int l : T.leaves ) S.leaf_min = std::min( S.leaf_min, T.c[ l ].n ), S.leaf_max = std::max( S.leaf_max, T.c[ l ].n );
+   for ( int l = 0; l < T.nleaves(); ++l ) {
+      const int c = T.cnt[ T.leaf_cluster( l ) ];
+      S.leaf_min = std::min( S.leaf_min, c ), S.leaf_max = std::max( S.leaf_max, c );
+   }
    for ( const Blk& b : blks ) {
       if ( b.type == kLowRank ) ++S.n_lowrank;
       else if ( b.small ) ++S.n_small;
@@ -1400,12 +1433,15 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
    const int n = in.n;
    double t0 = now();
    h.T = build_tree( n, in.h_v, in.h_xc, opt.leaf );
-   make_blocks( h.T, 0, 0, opt, h.blks );
+   h.blks = make_blocks( h.T, opt );
    const Tree& T = h.T;
    S.depth = T.depth;
-   S.nleaves = (int)T.leaves.size();
+   S.nleaves = T.nleaves();
    S.leaf_min = 1 << 30;
-   for ( int l : T.leaves ) S.leaf_min = std::min( S.leaf_min, T.c[ l ].n ), S.leaf_max = std::max( S.leaf_max, T.c[ l ].n );
+   for ( int l = 0; l < T.nleaves(); ++l ) {
+      const int c = T.cnt[ T.leaf_cluster( l ) ];
+      S.leaf_min = std::min( S.leaf_min, c ), S.leaf_max = std::max( S.leaf_max, c );
+   }
    if ( 3 * S.leaf_max > kMaxLeafRows ) throw std::runtime_error( "H-matrix: leaf clusters larger than 42 triangles" );
    S.t_tree = now() - t0;
    say( "cluster tree: depth %d, %d leaf clusters of %d to %d triangles, %.2f s", S.depth, S.nleaves, S.leaf_min, S.leaf_max,
@@ -1850,27 +1886,22 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
    }
 
    // per leaf cluster: the blocks covering its rows, in block order (fixed)
-   h.nleaves = (int)T.leaves.size();
-   std::vector<int> lidx( T.c.size(), -1 );
-   for ( int l = 0; l < h.nleaves; ++l ) lidx[ T.leaves[ l ] ] = l;
+   h.nleaves = T.nleaves();
    std::vector<std::vector<std::pair<int, int>>> rows( h.nleaves );
    std::vector<int> diag( h.nleaves, -1 );
    for ( size_t i = 0; i < h.blks.size(); ++i ) {
       const Blk& b = h.blks[ i ];
-      const Cluster& ct = T.c[ b.t ];
-      for ( int l = ct.leaf_first; l < ct.leaf_first + ct.leaf_count; ++l ) {
-         const Cluster& lc = T.c[ T.leaves[ l ] ];
-         rows[ l ].push_back( { (int)i, 3 * ( lc.beg - ct.beg ) } );
-      }
-      if ( b.t == b.s && ct.leaf() ) diag[ lidx[ b.t ] ] = (int)i;
+      int l0 = 0, nlf = 0;
+      T.leaf_range( b.t, l0, nlf );
+      for ( int l = l0; l < l0 + nlf; ++l ) rows[ l ].push_back( { (int)i, 3 * ( T.beg[ T.leaf_cluster( l ) ] - T.beg[ b.t ] ) } );
+      if ( b.t == b.s && T.is_leaf( b.t ) ) diag[ b.t - T.first_leaf() ] = (int)i;
    }
    std::vector<int> r0v( h.nleaves ), nrv( h.nleaves ), ptrv( h.nleaves + 1, 0 ), eb, eo;
    std::vector<long> poff( h.nleaves );
    long ptot = 0;
    for ( int l = 0; l < h.nleaves; ++l ) {
-      const Cluster& lc = T.c[ T.leaves[ l ] ];
-      r0v[ l ] = 3 * lc.beg;
-      nrv[ l ] = 3 * lc.n;
+      r0v[ l ] = 3 * T.beg[ T.leaf_cluster( l ) ];
+      nrv[ l ] = 3 * T.cnt[ T.leaf_cluster( l ) ];
       for ( auto& e : rows[ l ] ) eb.push_back( e.first ), eo.push_back( e.second );
       ptrv[ l + 1 ] = (int)eb.size();
       poff[ l ] = ptot;
