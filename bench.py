@@ -21,6 +21,7 @@ import argparse
 import importlib.util
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -47,23 +48,59 @@ def _package():
 
 
 def _cores():
+    """Threads for the CPU baseline: the CPUs this process may run on, capped by the CPU share
+    the host grants (OMP_NUM_THREADS where it is set; the GPU hosts set it to that share)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
 
 
-def cpu_baseline(tri, cores):
-    """The oracle on this host: one whole solve of the same mesh (C assembly with OpenMP, LAPACK
-    dgetrf/dgetrs through SciPy's OpenBLAS, C DIFCO/VISCO)."""
+def _host_cpu():
+    """CPU model and core counts from lscpu (falls back to /proc/cpuinfo)."""
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    model = info.get("Model name", "")
+    if not model:
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            model = ""
+    out = {"model": model}
+    try:
+        per = int(info.get("Core(s) per socket", "0")) * int(info.get("Socket(s)", "0"))
+        if per:
+            out["physical_cores"] = per
+        if info.get("CPU(s)"):
+            out["logical_cpus"] = int(info["CPU(s)"])
+    except ValueError:
+        pass
+    return out
+
+
+def cpu_baseline(tri, cores, runs=3):
+    """The oracle on this host: whole solves of the same mesh (C assembly with OpenMP, LAPACK
+    dgetrf/dgetrs through SciPy's OpenBLAS, C DIFCO/VISCO); the median wall time of `runs`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.lib()
-    t0 = time.perf_counter()
-    ref = O.best_run(tri.v1, tri.v2, tri.v3, tri.truearea, tri.truevol, iarea=1, cg=1.0, ivisco=1, icov=0)
-    el = time.perf_counter() - t0
-    return ref, el
+    times, ref = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        ref = O.best_run(tri.v1, tri.v2, tri.v3, tri.truearea, tri.truevol, iarea=1, cg=1.0, ivisco=1, icov=0)
+        times.append(time.perf_counter() - t0)
+    return ref, float(np.median(times)), times
 
 
 def hmatrix_c5(bh, M, steps):
@@ -330,7 +367,7 @@ def main():
         "phase_rooflines": phase_rooflines,
     }
     if not args.no_cpu:
-        ref, el = cpu_baseline(tri, cores)
+        ref, el, times = cpu_baseline(tri, cores)
         def err(a, b):
             return abs(a - b) / abs(b)
         errors = {}
@@ -339,10 +376,12 @@ def main():
                             "chiprime_eta": err(r["chiprime"], ref["chiprime"]),
                             "coupling_dtr": float(np.max(np.abs(r["dtr"] - ref["dtr"])) / np.max(np.abs(ref["dtr"])))}
         out["errors_vs_fp64_oracle"] = errors
+        host = _host_cpu()
         out["cpu_baseline"] = {"value": round(1.0 / el, 5), "unit": "solves/s", "cores": cores, "kind": "port",
-                               "sample": f"one whole {n}-triangle solve of the same mesh by the oracle (C assembly, "
-                                         f"OpenMP {cores} threads; LAPACK dgetrf/dgetrs via SciPy OpenBLAS; C DIFCO/VISCO): "
-                                         f"{el:.2f} s"}
+                               "host_cpu": host, "runs_s": [round(t, 3) for t in times],
+                               "sample": f"whole {n}-triangle solves of the same mesh by the oracle (C assembly, "
+                                         f"OpenMP {cores} threads; LAPACK dgetrf/dgetrs via SciPy OpenBLAS with "
+                                         f"{cores} threads; C DIFCO/VISCO), median of {len(times)}: {el:.2f} s"}
     s.close()
     if not args.no_series:
         out["series_c3"] = series_c3(bh, M, max(1, min(args.steps, 5)))

@@ -59,7 +59,8 @@ class BhStats(C.Structure):
     _fields_ = [("t_prep", C.c_double), ("t_assemble", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_reduce", C.c_double), ("t_total", C.c_double), ("ir_iterations", C.c_int),
                 ("ir_correction", C.c_double), ("pivoted", C.c_int), ("npad", C.c_int),
-                ("t_adjoint_solve", C.c_double), ("t_adjoint_pairs", C.c_double), ("t_adjoint_tri", C.c_double)]
+                ("t_adjoint_solve", C.c_double), ("t_adjoint_pairs", C.c_double), ("t_adjoint_tri", C.c_double),
+                ("ir_above_tol", C.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -136,6 +137,10 @@ def lib():
     L.bhg_factor_deflated.restype = i
     L.bhg_besolv.argtypes = [vp, i, _dp, lg]
     L.bhg_besolv.restype = i
+    L.bhg_set_option.argtypes = [vp, C.c_char_p, d]
+    L.bhg_set_option.restype = i
+    L.bhg_series_set_option.argtypes = [vp, C.c_char_p, d]
+    L.bhg_series_set_option.restype = i
     L.bhg_set_method.argtypes = [vp, i, d, i, d]
     L.bhg_set_method.restype = i
     L.bhg_set_hm_option.argtypes = [vp, C.c_char_p, d]
@@ -282,6 +287,10 @@ class Solver:
     def set_refinement(self, tol, maxit=30):
         self.L.bhg_set_refinement(self.h, float(tol), int(maxit))
 
+    def set_option(self, name, value):
+        """A setting of the dense path (bhg_set_option): 'fp32_growth', 'ir_steps_before_pivoting'."""
+        self._check(self.L.bhg_set_option(self.h, name.encode(), float(value)), f"set_option {name}")
+
     # -- the H-matrix path
     def set_method(self, method="dense", eps=1e-6, single=False, tol=0.0):
         """'dense' (LU) or 'hmatrix' (cross-approximated blocks + GMRES) for the next solve_mesh."""
@@ -405,6 +414,10 @@ class Series:
     def _check(self, rc, what):
         if rc != 0:
             raise HipError(f"{what}: {self.L.bhg_series_last_error(self.h).decode(errors='replace')}")
+
+    def set_option(self, name, value):
+        """bhg_set_option for every mesh (bhg_series_set_option)."""
+        self._check(self.L.bhg_series_set_option(self.h, name.encode(), float(value)), f"set_option {name}")
 
     def autotune(self, iarea=1, cg=1.0, ivisco=1):
         self._check(self.L.bhg_series_autotune(self.h, int(iarea), float(cg), int(ivisco)), "autotune")

@@ -42,31 +42,58 @@ __device__ inline double block_sum( double v, double* red ) {
 }
 
 // best/best.f:261-315 in one workgroup of 1024 threads: centroid of the triangle centroids,
-// shift, areas, total area and the area factor.  The sums are fixed-order tree sums (each thread
-// sums its stride in order), so they differ from the reference's running sums by rounding only.
+// shift, areas, total area and the area factor.  The two sums are the reference's running sums,
+// in its order and bit for bit: the workgroup computes a chunk of the terms (each exactly as the
+// reference computes it) into LDS, then one lane per coordinate adds them in triangle order.  The
+// pair integrals of nearly aligned triangle pairs are sensitive to the last bits of the shifted
+// coordinates (an entry moved by 2e-8 relative when the centroid differed by 6e-15 A), so the
+// shift must be the reference's exactly.
 __global__ void __launch_bounds__( 1024 ) k_prep( int n, double* v1, double* v2, double* v3, int iarea, double truearea,
                                                  double* area, double* scal /* [0]=afac [1]=totarea [2..4]=centroid */ ) {
-   __shared__ double red[ 1024 ];
-   double cs[ 3 ] = { 0.0, 0.0, 0.0 };
-   for ( int j = threadIdx.x; j < n; j += blockDim.x )
-      for ( int k = 0; k < 3; ++k ) cs[ k ] = cs[ k ] + ( v1[ k * n + j ] + v2[ k * n + j ] + v3[ k * n + j ] ) / 3.0;
-   double cen[ 3 ];
-   for ( int k = 0; k < 3; ++k ) cen[ k ] = block_sum( cs[ k ], red ) / (double)n;
-   double tot = 0.0;
-   for ( int i = threadIdx.x; i < n; i += blockDim.x ) {
-      for ( int k = 0; k < 3; ++k ) {
-         v1[ k * n + i ] = v1[ k * n + i ] - cen[ k ];
-         v2[ k * n + i ] = v2[ k * n + i ] - cen[ k ];
-         v3[ k * n + i ] = v3[ k * n + i ] - cen[ k ];
+   constexpr int kChunk = 1024;
+   __shared__ double term[ 3 * kChunk ];
+   __shared__ double shared_sum[ 3 ];
+   double run = 0.0;  // lanes 0..2: the running sum of coordinate threadIdx.x
+   for ( int j0 = 0; j0 < n; j0 += kChunk ) {
+      const int j = j0 + threadIdx.x;
+      if ( j < n )
+         for ( int k = 0; k < 3; ++k ) term[ k * kChunk + threadIdx.x ] = ( v1[ k * n + j ] + v2[ k * n + j ] + v3[ k * n + j ] ) / 3.0;
+      __syncthreads();
+      if ( threadIdx.x < 3 ) {
+         const int m = min( kChunk, n - j0 );
+         for ( int q = 0; q < m; ++q ) run = run + term[ threadIdx.x * kChunk + q ];
       }
-      const double ax = v2[ i ] - v1[ i ], ay = v2[ n + i ] - v1[ n + i ], az = v2[ 2 * n + i ] - v1[ 2 * n + i ];
-      const double bx = v3[ i ] - v1[ i ], by = v3[ n + i ] - v1[ n + i ], bz = v3[ 2 * n + i ] - v1[ 2 * n + i ];
-      const double px = by * az - bz * ay, py = bz * ax - bx * az, pz = bx * ay - by * ax;
-      const double ar = sqrt( px * px + py * py + pz * pz ) * 0.5;
-      area[ i ] = ar;
-      tot = tot + ar;
+      __syncthreads();
    }
-   tot = block_sum( tot, red );
+   if ( threadIdx.x < 3 ) shared_sum[ threadIdx.x ] = run / (double)n;
+   __syncthreads();
+   const double cen[ 3 ] = { shared_sum[ 0 ], shared_sum[ 1 ], shared_sum[ 2 ] };
+   double tot = 0.0;  // lane 0: the running total area
+   for ( int i0 = 0; i0 < n; i0 += kChunk ) {
+      const int i = i0 + threadIdx.x;
+      if ( i < n ) {
+         for ( int k = 0; k < 3; ++k ) {
+            v1[ k * n + i ] = v1[ k * n + i ] - cen[ k ];
+            v2[ k * n + i ] = v2[ k * n + i ] - cen[ k ];
+            v3[ k * n + i ] = v3[ k * n + i ] - cen[ k ];
+         }
+         const double ax = v2[ i ] - v1[ i ], ay = v2[ n + i ] - v1[ n + i ], az = v2[ 2 * n + i ] - v1[ 2 * n + i ];
+         const double bx = v3[ i ] - v1[ i ], by = v3[ n + i ] - v1[ n + i ], bz = v3[ 2 * n + i ] - v1[ 2 * n + i ];
+         const double px = by * az - bz * ay, py = bz * ax - bx * az, pz = bx * ay - by * ax;
+         const double ar = sqrt( px * px + py * py + pz * pz ) * 0.5;
+         area[ i ] = ar;
+         term[ threadIdx.x ] = ar;
+      }
+      __syncthreads();
+      if ( threadIdx.x == 0 ) {
+         const int m = min( kChunk, n - i0 );
+         for ( int q = 0; q < m; ++q ) tot = tot + term[ q ];
+      }
+      __syncthreads();
+   }
+   if ( threadIdx.x == 0 ) shared_sum[ 0 ] = tot;
+   __syncthreads();
+   tot = shared_sum[ 0 ];
    const double afac = iarea == 1 ? truearea / tot : 1.0;
    if ( threadIdx.x == 0 ) {
       scal[ 0 ] = afac;

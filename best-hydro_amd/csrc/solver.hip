@@ -91,6 +91,13 @@ struct PhaseTimer {
    }
 };
 
+// Work vectors of one solve (solve_device and the refinement): the solver's own buffers, or the
+// static work space of bhg_besolv_noalloc.
+struct SolveWork {
+   double *b, *x, *r, *y64, *x64il, *p64, *part, *norms;
+   float *d32, *y32, *x32, *p32;
+};
+
 }  // namespace bhg
 
 struct bhg_solver {
@@ -142,6 +149,10 @@ struct bhg_solver {
    bhg_stats stats {};
    double ir_tol = 1e-12;
    int ir_maxit = 30;
+   double fp32_growth = bhg::kFp32Growth;  // largest multiplier the exchange-free FP32 LU accepts
+   int ir_steps_nopiv = 0;                 // refinement steps with those factors before the pivoted
+                                           // refactorization (0: ir_maxit)
+   const bhg::SolveWork* work_override = nullptr;  // solve_device's work vectors when not the buffers above
 
    void say( const char* fmt, ... ) __attribute__( ( format( printf, 2, 3 ) ) ) {
       char buf[ 512 ];
@@ -271,11 +282,11 @@ void factor( bhg_solver* s, int precision ) {
       s->a32.ensure( np * np );
       s->inv32.ensure( (size_t)nblk * 2 * kNB * kNB );
       if ( !fresh32 ) convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
-      g_growth = kFp32Growth;
+      g_growth = s->fp32_growth;
       lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
       g_growth = 1.0;
       if ( read_flag( s ) ) {
-         s->say( "LU (FP32): a multiplier exceeded 256; refactoring with row exchanges" );
+         s->say( "LU (FP32): a multiplier exceeded %g; refactoring with row exchanges", s->fp32_growth );
          convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
          fill( s->st, s->flags.p, 4, 0 );
          lu_pivoted<float>( s->st, s->npad, s->a32.p, (long)np, s->inv32.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
@@ -343,6 +354,13 @@ void size_vectors( bhg_solver* s ) {
    }
 }
 
+// The work vectors solve_device uses.
+SolveWork work( bhg_solver* s ) {
+   if ( s->work_override ) return *s->work_override;
+   return SolveWork { s->b.p, s->x.p, s->r.p, s->y64.p, s->x64il.p, s->p64.p, s->part.p, s->norms.p,
+                      s->d32.p, s->y32.p, s->x32.p, s->p32.p };
+}
+
 // How lu_solve applies the row exchanges of the resident factors.
 int piv_mode( const bhg_solver* s ) { return s->pivoted ? ( s->transposed ? -1 : 1 ) : 0; }
 
@@ -351,18 +369,19 @@ int piv_mode( const bhg_solver* s ) { return s->pivoted ? ( s->transposed ? -1 :
 // previous step's.  No host interaction.
 void ir_step( bhg_solver* s, int nrhs, int it ) {
    const long np = s->npad;
-   hipLaunchKernelGGL( k_ir_shift, dim3( 1 ), dim3( 64 ), 0, s->st, s->norms.p );
+   const SolveWork w = work( s );
+   hipLaunchKernelGGL( k_ir_shift, dim3( 1 ), dim3( 64 ), 0, s->st, w.norms );
    if ( it == 0 ) {
-      fill( s->st, s->x.p, np * nrhs, 0.0 );
-      to_interleaved<double, float>( s->st, s->npad, nrhs, s->b.p, np, s->d32.p );
+      fill( s->st, w.x, np * nrhs, 0.0 );
+      to_interleaved<double, float>( s->st, s->npad, nrhs, w.b, np, w.d32 );
    } else {
-      residual( s->st, s->npad, s->n3, nrhs, s->a64.p, np, s->x.p, np, s->b.p, np, s->r.p, np, s->part.p, s->norms.p );
-      to_interleaved<double, float>( s->st, s->npad, nrhs, s->r.p, np, s->d32.p );
-      fill( s->st, s->norms.p, 32, 0.0 );
+      residual( s->st, s->npad, s->n3, nrhs, s->a64.p, np, w.x, np, w.b, np, w.r, np, w.part, w.norms );
+      to_interleaved<double, float>( s->st, s->npad, nrhs, w.r, np, w.d32 );
+      fill( s->st, w.norms, 32, 0.0 );
    }
-   lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, piv_mode( s ), s->d32.p, s->y32.p,
-                    s->x32.p, s->p32.p );
-   ir_update( s->st, s->n3, nrhs, s->x.p, np, s->x32.p, np, s->norms.p );
+   lu_solve<float>( s->st, s->npad, s->n3, kSuper, s->a32.p, np, s->xl32.p, s->xu32.p, s->ipiv.p, piv_mode( s ), w.d32, w.y32,
+                    w.x32, w.p32 );
+   ir_update( s->st, s->n3, nrhs, w.x, np, w.x32, np, w.norms );
 }
 
 // The last refinement step, per column k: the correction c_k = max |dx| / max |x| and, from the
@@ -375,7 +394,7 @@ struct IrCorrection {
 };
 IrCorrection ir_correction( bhg_solver* s, int nrhs ) {
    double hn[ 64 ];
-   BH_HIP( hipMemcpyAsync( hn, s->norms.p, sizeof hn, hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipMemcpyAsync( hn, work( s ).norms, sizeof hn, hipMemcpyDeviceToHost, s->st ) );
    BH_HIP( hipStreamSynchronize( s->st ) );
    IrCorrection r;
    for ( int k = 0; k < nrhs; ++k ) {
@@ -393,14 +412,15 @@ IrCorrection ir_correction( bhg_solver* s, int nrhs ) {
 // last corrections, below tol (step 0's correction is 1, the whole of x0).
 bool ir_converged( const IrCorrection& c, int it, double tol ) { return it > 0 && ( c.worst < tol || c.est < tol ); }
 
-// FP64 iterative refinement with the FP32 factors from step it0 on, until the correction, or the
-// error estimated to remain, is below ir_tol relative to x in every column (or the correction
-// stagnates at the FP64 floor).  Returns false if the correction stagnated above kIrFloor (the
-// FP32 factors too poor for the refinement to converge).
+// FP64 iterative refinement with the FP32 factors from step it0 up to step maxit, until the
+// correction, or the error estimated to remain, is below ir_tol relative to x in every column.
+// A refinement that stagnates (the correction no longer halves) below kIrFloor is accepted: the
+// FP64 residuals themselves then limit it; the log says so and bhg_stats.ir_above_tol is set.
+// Returns false if it stopped above kIrFloor (the FP32 factors too poor for the refinement).
 constexpr double kIrFloor = 1e-10;
-bool ir_loop( bhg_solver* s, int nrhs, int it0 ) {
+bool ir_loop( bhg_solver* s, int nrhs, int it0, int maxit ) {
    double prev = 1e300, worst = 1e300;
-   for ( int it = it0; it < s->ir_maxit; ++it ) {
+   for ( int it = it0; it < maxit; ++it ) {
       ir_step( s, nrhs, it );
       const IrCorrection c = ir_correction( s, nrhs );
       worst = c.worst;
@@ -410,8 +430,15 @@ bool ir_loop( bhg_solver* s, int nrhs, int it0 ) {
       if ( it > 2 && worst > 0.5 * prev ) break;  // stagnation
       prev = worst;
    }
-   return worst < kIrFloor;
+   if ( worst < kIrFloor ) {
+      s->stats.ir_above_tol = 1;
+      s->say( "refinement: stagnated at a correction of %.2e, above the tolerance %.0e; accepted (below %.0e)", worst,
+              s->ir_tol, kIrFloor );
+      return true;
+   }
+   return false;
 }
+bool ir_loop( bhg_solver* s, int nrhs, int it0 ) { return ir_loop( s, nrhs, it0, s->ir_maxit ); }
 
 // The FP32 factors of the resident (deflated) matrix with row exchanges, after the exchange-free
 // ones did not let the refinement converge.
@@ -425,26 +452,31 @@ void refactor_pivoted32( bhg_solver* s ) {
    build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, (long)np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
 }
 
-// Solves for the nrhs columns of s->b (npad x nrhs, rows >= n3 zero) into s->x.
+// Solves for the nrhs columns of b (npad x nrhs, rows >= n3 zero) into x (the work vectors:
+// s->b and s->x unless overridden).
 void solve_device( bhg_solver* s, int nrhs ) {
    const long np = s->npad;
+   const SolveWork w = work( s );
    s->stats.ir_iterations = 0;
    s->stats.ir_correction = 0.0;
+   s->stats.ir_above_tol = 0;
    if ( s->factored == -1 ) {
+      if ( s->work_override ) throw std::runtime_error( "the H-matrix solve needs its own work space" );
       std::vector<std::string> lines;
       hm_solve( *s->hm, s->st, nrhs, s->b.p, np, s->x.p, np, &s->hm_stats, &lines );
       for ( auto& l : lines ) s->log.push_back( std::move( l ) );
       return;
    }
    if ( s->factored == 64 ) {
-      to_interleaved<double, double>( s->st, s->npad, nrhs, s->b.p, np, s->r.p );
-      lu_solve<double>( s->st, s->npad, s->n3, kSuper, s->a64.p, np, s->xl64.p, s->xu64.p, s->ipiv.p, piv_mode( s ), s->r.p,
-                        s->y64.p, s->x64il.p, s->p64.p );
-      from_interleaved<double, double>( s->st, s->n3, nrhs, s->x64il.p, s->x.p, np );
+      to_interleaved<double, double>( s->st, s->npad, nrhs, w.b, np, w.r );
+      lu_solve<double>( s->st, s->npad, s->n3, kSuper, s->a64.p, np, s->xl64.p, s->xu64.p, s->ipiv.p, piv_mode( s ), w.r,
+                        w.y64, w.x64il, w.p64 );
+      from_interleaved<double, double>( s->st, s->n3, nrhs, w.x64il, w.x, np );
       return;
    }
    if ( s->factored != 32 ) throw std::runtime_error( "solve before the factorization" );
-   if ( ir_loop( s, nrhs, 0 ) ) return;
+   const int first = s->ir_steps_nopiv > 0 && !s->pivoted ? std::min( s->ir_steps_nopiv, s->ir_maxit ) : s->ir_maxit;
+   if ( ir_loop( s, nrhs, 0, first ) ) return;
    if ( s->pivoted || s->transposed ) throw std::runtime_error( "iterative refinement did not converge" );
    refactor_pivoted32( s );
    if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
@@ -556,7 +588,7 @@ void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshT
    s->deflated = true;
    s->a32_fresh = false;
    fill( s->st, s->flags.p, 4, 0 );
-   g_growth = kFp32Growth;
+   g_growth = s->fp32_growth;
    lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
    g_growth = 1.0;
    build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
@@ -575,8 +607,9 @@ void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
    s->pivoted = false;
    s->transposed = false;
    s->mesh_ready = true;
+   s->stats.ir_above_tol = 0;
    if ( read_flag( s ) ) {
-      s->say( "series: a multiplier exceeded 256; the mesh is solved again with row exchanges" );
+      s->say( "series: a multiplier exceeded %g; the mesh is solved again with row exchanges", s->fp32_growth );
       assemble( s, true, true );
       factor( s, 32 );
       launch_rhs( s->st, s->n, s->xc.p, s->b.p, s->npad, ivisco ? 1 : 0 );
@@ -1094,6 +1127,20 @@ int bhg_set_refinement( bhg_solver* s, double tol, int maxit ) {
    return 0;
 }
 
+int bhg_set_option( bhg_solver* s, const char* name, double value ) {
+   if ( !s || !name ) return 1;
+   const std::string k( name );
+   if ( k == "fp32_growth" && value >= 1.0 ) {
+      s->fp32_growth = value;
+   } else if ( k == "ir_steps_before_pivoting" && value >= 0.0 ) {
+      s->ir_steps_nopiv = (int)value;
+   } else {
+      s->last_error = "bhg_set_option: unknown option or value out of range: " + k;
+      return 1;
+   }
+   return 0;
+}
+
 int bhg_set_method( bhg_solver* s, int method, double eps, int single, double tol ) {
    if ( !s || ( method != 0 && method != 1 ) ) return 1;
    s->method = method;
@@ -1324,6 +1371,7 @@ int bhg_besolv( bhg_solver* s, int nrhs, double* bh, long ldb ) {
    } );
 }
 
+
 // ---------------------------------------------------------------------- hmatrix/best_hmatrix.h
 
 // ------------------------------------------------------------------ series of meshes
@@ -1469,7 +1517,17 @@ int bhg_series_set_tuning( bhg_series* S, int k, int strip, int gemm_bk, int ir_
       if ( k < 0 || k >= (int)S->m.size() ) throw std::runtime_error( "mesh index out of range" );
       if ( strip < 1 || ( gemm_bk != 0 && gemm_bk != 16 && gemm_bk != 32 && gemm_bk != 64 ) || ir_iters < 1 )
          throw std::runtime_error( "strip >= 1, gemm_bk 0, 16, 32 or 64, ir_iters >= 1" );
+      // the eager continuation after the replay needs steps left below the refinement limit
+      if ( ir_iters >= S->m[ k ]->ir_maxit ) throw std::runtime_error( "ir_iters must be below the refinement step limit" );
       S->tune[ k ] = bhg::MeshTune { strip, gemm_bk, ir_iters };
+      S->drop_graph();
+   } );
+}
+
+int bhg_series_set_option( bhg_series* S, const char* name, double value ) {
+   return bhg::series_guarded( S, "bhg_series_set_option", [&] {
+      for ( auto* m : S->m )
+         if ( bhg_set_option( m, name, value ) != 0 ) throw std::runtime_error( m->last_error );
       S->drop_graph();
    } );
 }
@@ -1501,7 +1559,7 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
          bhg::MeshTune t;
          // an eager solve: allocations, and the refinement steps this mesh needs
          if ( bhg_solve_mesh( m, iarea, cg, ivisco, 0, 32, nullptr ) != 0 ) throw std::runtime_error( m->last_error );
-         t.ir_iters = std::max( 2, m->stats.ir_iterations );
+         t.ir_iters = std::min( std::max( 2, m->stats.ir_iterations ), m->ir_maxit - 1 );
          bhg::prepare_buffers( m );
          const long np = m->npad;
          auto time = [&]( auto&& f ) {
@@ -1533,7 +1591,7 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
                bhg::g_gemm_bk_f32 = bk;
                bhg::fill( m->st, m->flags.p, 4, 0 );
                bhg::convert_d2f( m->st, np * np, m->a64.p, m->a32.p );
-               bhg::g_growth = bhg::kFp32Growth;
+               bhg::g_growth = m->fp32_growth;
                bhg::lu_nopiv<float>( m->lu_streams(), m->npad, m->a32.p, np, m->inv32.p, m->flags.p );
                bhg::g_growth = 1.0;
                bhg::build_super_inverses<float>( m->st, m->npad, bhg::kSuper, m->a32.p, np, m->inv32.p, m->xl32.p, m->xu32.p,

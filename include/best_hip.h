@@ -88,6 +88,8 @@ typedef struct bhg_stats {
    int npad;             /* padded order of the matrix */
    double t_adjoint_solve, t_adjoint_pairs, t_adjoint_tri; /* bhg_gradient: the transposed solves
                                                               (3 x 11 RHS), the pair term, the rest */
+   int ir_above_tol;     /* 1 if the refinement stagnated below 1e-10 but above its tolerance and the
+                            solution was accepted there (the log says so) */
 } bhg_stats;
 
 bhg_solver* bhg_create( int device );
@@ -125,6 +127,13 @@ int bhg_gradient( bhg_solver* s, int iarea, double cg, int precision, bhg_result
 int bhg_get_solution( bhg_solver* s, int nrhs, double* out, long ldo );
 int bhg_get_stats( bhg_solver* s, bhg_stats* out );
 int bhg_set_refinement( bhg_solver* s, double tol, int maxit );
+/* Settings of the dense path that BEST does not have (it calls LAPACK's partial pivoting):
+ * "fp32_growth" (256): the largest multiplier the exchange-free FP32 LU accepts before it
+ * refactors with row exchanges (the FP64 LU keeps partial pivoting's own test, 1);
+ * "ir_steps_before_pivoting" (0 = the refinement step limit): refinement steps allowed with the
+ * exchange-free FP32 factors before the matrix is refactored with row exchanges.  Both exist so
+ * that the fallbacks can be exercised.  Returns 1 for an unknown name or a value out of range. */
+int bhg_set_option( bhg_solver* s, const char* name, double value );
 
 /* Launch profiling of bhg_solve_mesh: with on = 1 every launch of the timed kernel classes is
  * bracketed by HIP events on the solver's stream.  bhg_get_profile returns, per class (0 pair
@@ -237,6 +246,8 @@ int bhg_series_autotune( bhg_series* s, int iarea, double cg, int ivisco );
 /* gemm_bk: 16, 32 or 64 on every tile, or 0 (a single solve's default): 16 on the 128 x 128 tiles
  * of the trailing updates and 32 on the smaller ones. */
 int bhg_series_set_tuning( bhg_series* s, int k, int strip, int gemm_bk, int ir_iters );
+/* bhg_set_option for every mesh of the series (the schedule is captured again). */
+int bhg_series_set_option( bhg_series* s, const char* name, double value );
 /* out[0..3] = strip, gemm_bk, ir_iters, side by side (1) or chained (0). */
 int bhg_series_get_tuning( bhg_series* s, int k, int* out );
 

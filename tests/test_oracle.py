@@ -329,3 +329,60 @@ def test_compon_surfaces_and_open_edges(oracle, bh):
     m2 = oracle.prep(t.v1[1:], t.v2[1:], t.v3[1:], 1, t.truearea)
     _, nc2, no2 = oracle.compon(m2)
     assert no2 == 3
+
+
+# ------------------------------------------------------------------ the transport algebra (best/difco.f, best/jacobi.f)
+
+@pytest.mark.parametrize("which", ["bumpy320", "2eu7_02562", "1aki49_atom_01806"])
+def test_transport_algebra_fixed_by_mathematics(oracle, which):
+    """DIFCO's tensors, centres and JACOBI's eigenvalues against what the mathematics fixes
+    (tests/transport_checks.py): the 6 x 6 friction inverse moved to the centre of diffusion, the
+    centres as the points of symmetric coupling, LAPACK's eigenvalues, exact scaling by 2 (and by 3
+    to rounding) without the area correction, and the cyclic rotation of the coordinates.  These
+    pin the oracle's restatement of the algebra below the reference fixtures' printed digits."""
+    import transport_checks as TC
+    from best_hydro_amd import meshes as M
+    t = M.bumpy_ellipsoid_mesh(level=2) if which == "bumpy320" else M.read_best(os.path.join(GOLDEN, which))
+    run = lambda a, b, c, ta, tv, **kw: oracle.best_run(a, b, c, ta, tv, ivisco=1, **kw)
+    r = run(t.v1, t.v2, t.v3, t.truearea, t.truevol)
+    mob = TC.mobility_errors(r)
+    assert mob["dtr"] < 1e-13 and mob["drr"] < 1e-13 and mob["dtt"] < mob["dtt_bound"], mob
+    e_cr, b_cr, e_cd, b_cd = TC.centre_errors(r)
+    assert e_cr < b_cr and e_cd < b_cd, (e_cr, b_cr, e_cd, b_cd)
+    for k, v in TC.eigen_errors(r).items():
+        assert v < 1e-13, k
+    r1 = run(t.v1, t.v2, t.v3, 0.0, 0.0, iarea=0)
+    for s, bar in ((2.0, 0.0), (3.0, 1e-10)):
+        rs = run(t.v1 * s, t.v2 * s, t.v3 * s, 0.0, 0.0, iarea=0)
+        for k, v in TC.scaling_errors(r1, rs, s).items():
+            assert v <= bar, (s, k, v)
+    rr = run(TC.cyclic(t.v1), TC.cyclic(t.v2), TC.cyclic(t.v3), t.truearea, t.truevol)
+    for k, v in TC.covariance_errors(r, rr).items():
+        assert v < 1e-8, k
+
+
+def test_transport_checks_catch_planted_mistakes(oracle):
+    """Plausible slips in the algebra fail the checks above: a sign of the centre of diffusion, the
+    coupling moved to a centre of resistance with two components exchanged, Dtr transposed before
+    the move to the centre of diffusion, eigenvalues of the lower instead of the upper triangle."""
+    import transport_checks as TC
+    from best_hydro_amd import meshes as M
+    t = M.bumpy_ellipsoid_mesh(level=2)
+    r = oracle.best_run(t.v1, t.v2, t.v3, t.truearea, t.truevol, ivisco=1)
+    bad = dict(r, udr=-r["udr"])
+    m = TC.mobility_errors(bad)
+    assert m["dtt"] > m["dtt_bound"] or m["dtr"] > 1e-13
+    # the coupling moved to a centre of resistance with two components exchanged
+    k0 = r["krt"] - r["ktt"] @ TC.cross_matrix(r["urc"])
+    u_bad = r["urc"][[1, 0, 2]]
+    bad = dict(r, urc=u_bad, krt=k0 + r["ktt"] @ TC.cross_matrix(u_bad))
+    e_cr, b_cr, _, _ = TC.centre_errors(bad)
+    assert e_cr > b_cr
+    u = TC.cross_matrix(r["udr"])
+    d_cr = r["dtr"] + u @ r["drr"]
+    bad = dict(r, dtr=d_cr.T - u @ r["drr"])
+    assert TC.mobility_errors(bad)["dtr"] > 1e-13
+    a = r["drr"]
+    low = np.tril(a) + np.tril(a, -1).T
+    bad = dict(r, drr_eig=np.linalg.eigvalsh(low))
+    assert TC.eigen_errors(bad)["drr_eig"] > 1e-13
