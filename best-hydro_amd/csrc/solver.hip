@@ -13,6 +13,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -191,6 +192,30 @@ __global__ void k_ir_shift( double* norms ) {
    if ( k < 32 ) {
       norms[ 32 + k ] = norms[ k ];
       norms[ k ] = 0.0;
+   }
+}
+
+// ------------------------------------------------------------------ the static work space of
+// bhg_besolv_noalloc: every work vector of one solve (the FP64 right-hand sides, solutions and
+// residuals, the split-K partial sums of the triangular solves in both precisions, the residual
+// product's partial sums, the refinement's norms) for matrices of order up to kNoAllocMaxPad and
+// up to kMaxRhs right-hand sides, in one file-scope device array (89 MiB on each GPU the library
+// runs on, held until the process ends).  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (noalloc_scratch below).
+constexpr long kNoAllocMaxPad = 16384;
+constexpr long kNaVec = kNoAllocMaxPad * kMaxRhs;                      // one npad x 16 block
+constexpr long kNaPart = ( kNoAllocMaxPad / 512 ) * 16 * kNoAllocMaxPad;  // residual partials at that order
+constexpr size_t kNaDoubles = 9 * (size_t)kNaVec + (size_t)kNaPart + 64;  // b x r y64 x64il, p64 (4), part, norms
+constexpr size_t kNaFloats = 7 * (size_t)kNaVec;                          // d32 y32 x32, p32 (4)
+constexpr size_t kNaBytes = ( ( kNaDoubles * sizeof( double ) + kNaFloats * sizeof( float ) ) + 255 ) / 256 * 256;
+static __device__ __align__( 256 ) char scratch[ kNaBytes ];
+
+// b (npad x nrhs, leading dimension npad) = the caller's n3 x nrhs block, zero below row n3.
+__global__ void k_noalloc_copy_in( int n3, int npad, int nrhs, const double* __restrict__ src, long lds, double* __restrict__ b ) {
+   const long total = (long)npad * nrhs;
+   for ( long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x ) {
+      const long i = e % npad, j = e / npad;
+      b[ e ] = i < n3 ? src[ j * lds + i ] : 0.0;
    }
 }
 
@@ -1371,6 +1396,70 @@ int bhg_besolv( bhg_solver* s, int nrhs, double* bh, long ldb ) {
    } );
 }
 
+}  // extern "C"
+
+namespace bhg {
+namespace {
+
+// The device address of `scratch` on the current device, looked up once per device.
+char* noalloc_scratch( int device ) {
+   static std::mutex m;
+   static std::array<char*, 64> ptr {};
+   if ( device < 0 || device >= (int)ptr.size() ) throw std::runtime_error( "device index out of range" );
+   std::lock_guard<std::mutex> lk( m );
+   if ( !ptr[ device ] ) {
+      void* p = nullptr;
+      BH_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ) );
+      if ( !p ) throw std::runtime_error( "hipGetSymbolAddress returned no address for the work space" );
+      ptr[ device ] = static_cast<char*>( p );
+   }
+   return ptr[ device ];
+}
+
+}  // namespace
+}  // namespace bhg
+
+extern "C" {
+
+int bhg_besolv_noalloc( bhg_solver* s, int nrhs, const double* d_b, long ldb, double* d_x, long ldx ) {
+   return guarded( s, "bhg_besolv_noalloc", [&] {
+      if ( s->factored != 32 && s->factored != 64 )
+         throw std::runtime_error( s->factored == -1 ? "the H-matrix solve needs its own work space (use bhg_besolv)"
+                                                     : "no factorization" );
+      if ( nrhs < 1 || nrhs > bhg::kMaxRhs ) throw std::runtime_error( "between 1 and 16 right-hand sides per call" );
+      if ( s->npad > bhg::kNoAllocMaxPad ) {
+         char buf[ 160 ];
+         std::snprintf( buf, sizeof buf, "matrix order %d (padded) exceeds the %ld the static work space holds (use bhg_besolv)",
+                        s->npad, bhg::kNoAllocMaxPad );
+         throw std::runtime_error( buf );
+      }
+      if ( ldb < s->n3 || ldx < s->n3 ) throw std::runtime_error( "leading dimensions below 3 n" );
+      if ( !d_b || !d_x ) throw std::runtime_error( "null vector" );
+      bhg::set_device( s );
+      char* base = bhg::noalloc_scratch( s->device );
+      const long np = s->npad;
+      double* d = reinterpret_cast<double*>( base );
+      bhg::SolveWork w {};
+      w.b = d, w.x = d + bhg::kNaVec, w.r = d + 2 * bhg::kNaVec, w.y64 = d + 3 * bhg::kNaVec, w.x64il = d + 4 * bhg::kNaVec;
+      w.p64 = d + 5 * bhg::kNaVec;
+      w.part = d + 9 * bhg::kNaVec;
+      w.norms = w.part + bhg::kNaPart;
+      float* f = reinterpret_cast<float*>( base + bhg::kNaDoubles * sizeof( double ) );
+      w.d32 = f, w.y32 = f + bhg::kNaVec, w.x32 = f + 2 * bhg::kNaVec, w.p32 = f + 3 * bhg::kNaVec;
+      if ( bhg::residual_part_elems( s->npad ) > (size_t)bhg::kNaPart ) throw std::runtime_error( "residual work space" );
+      hipLaunchKernelGGL( bhg::k_noalloc_copy_in, dim3( 1024 ), dim3( 256 ), 0, s->st, s->n3, s->npad, nrhs, d_b, ldb, w.b );
+      bhg::fill( s->st, w.norms, 64, 0.0 );
+      struct Restore {
+         bhg_solver* s;
+         ~Restore() { s->work_override = nullptr; }
+      } restore { s };
+      s->work_override = &w;
+      bhg::solve_device( s, nrhs );
+      BH_HIP( hipMemcpy2DAsync( d_x, ldx * sizeof( double ), w.x, np * sizeof( double ), s->n3 * sizeof( double ), nrhs,
+                                hipMemcpyDeviceToDevice, s->st ) );
+      BH_HIP( hipStreamSynchronize( s->st ) );
+   } );
+}
 
 // ---------------------------------------------------------------------- hmatrix/best_hmatrix.h
 

@@ -160,6 +160,18 @@ int bhg_factor_deflated( bhg_solver* s, int ndef, const double* gdef, const int*
 /* BESOLV twin (best/besolve.f90:95-134): b (3n x nrhs, leading dimension ldb) in place. */
 int bhg_besolv( bhg_solver* s, int nrhs, double* b, long ldb );
 
+/* bhg_besolv for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  b and x are
+ * DEVICE pointers (3n x nrhs, leading dimensions ldb and ldx); the solve's work vectors live in
+ * one static device array of the library (89 MiB on each GPU it runs on, held until the process
+ * ends), so nothing is allocated or freed: the resident factors of bhg_solve_mesh or
+ * bhg_factor_deflated (FP64 LU, or FP32 LU + FP64 refinement) are used as they are.  Limits: padded
+ * matrix order up to 16,384 (5,461 triangles) and up to 16 right-hand sides, else nonzero with a
+ * message; not the H-matrix path (its Krylov basis needs its own memory).  Calls that overlap on
+ * one GPU share that array and race; the solver's own solution (bhg_get_solution) is not updated.
+ * Synchronizes its stream before returning. */
+int bhg_besolv_noalloc( bhg_solver* s, int nrhs, const double* b, long ldb, double* x, long ldx );
+
 /* ------------------------------------------------------------------ the H-matrix path */
 
 /* Diagnostics of the last H-matrix build and solve (the figures hmatrix/src/api.cpp:380-386 and
