@@ -24,6 +24,12 @@ struct HmOptions {
    int restart = 120;          // GMRES restart length (the reference's 30 with its H-LU; 120 with block Jacobi)
    int maxsteps = 500;         // GMRES steps (batched products) per solve
    double ws_bytes = 24e9;     // device work space of one batch of cross approximations
+   int precond = 0;            // GMRES preconditioner: 0 block Jacobi (then the hierarchical factorization at
+                               //   eps_pre if GMRES stalls), 1 the hierarchical factorization at eps_pre
+   double eps_pre = 0.1;       // its accuracy (BEST's EPSPRE, best/besolve.f90:34)
+   int pre_skip = 0;           // coarsest levels of the hierarchical factorization left out (diagnostic)
+   int direct = 0;             // 1 (-hlu): the hierarchical factorization at eps solves directly, refined
+                               //   against the H-matrix, no GMRES
 };
 
 struct HmStats {
@@ -42,6 +48,10 @@ struct HmStats {
    long n_trunc_failed = 0;    // approximations left as they were (dependent columns, or rank above the limit)
    double kmean_aca = 0;       // mean rank before the recompression
    double t_trunc = 0;         // seconds in the recompression (part of t_fill)
+   double eps_pre = 0;         // accuracy of the hierarchical preconditioner (0: block Jacobi)
+   double piv_ratio = 0;       // its smallest pivot ratio (cluster systems)
+   int pre_levels = 0, pre_rank_max = 0;
+   double pre_bytes = 0;
 };
 
 // The geometry: vertices (v1, v2, v3 as BEST's NAR x 3 column-major arrays, one after the other)

@@ -35,6 +35,7 @@
 #include <cstring>
 #include <deque>
 #include <numeric>
+#include <unordered_map>
 
 #include "bh_common.h"
 #include "hmatrix.h"
@@ -879,10 +880,11 @@ __global__ void __launch_bounds__( NT ) k_hm_trunc( const TruncJob* __restrict__
    if ( tid == 0 ) rank[ blockIdx.x ] = r, failed[ blockIdx.x ] = 0;
 }
 
-// U' = U Cu (column-major, 3m x r) and V' = V Cv (stored with the rank index fastest) into the
-// block's storage (E).  One workgroup per (job, side, chunk of kTrApply rows), one row per thread,
-// 16 columns of the result per pass with that slice of C staged in LDS.
-template <typename E>
+// U' = U Cu (column-major, 3m x r) and V' = V Cv (stored with the rank index fastest, or with
+// kVcol column-major like U) into the block's storage (E).  One workgroup per (job, side, chunk of
+// kTrApply rows), one row per thread, 16 columns of the result per pass with that slice of C
+// staged in LDS.
+template <typename E, bool kVcol = false>
 __global__ void __launch_bounds__( kTrApply ) k_hm_trunc_apply( const int4* __restrict__ tasks, const TruncJob* __restrict__ jobs,
                                                              const int* __restrict__ rank, void* const* __restrict__ uout,
                                                              void* const* __restrict__ vout ) {
@@ -912,7 +914,7 @@ __global__ void __launch_bounds__( kTrApply ) k_hm_trunc_apply( const int4* __re
 #pragma unroll
          for ( int c = 0; c < 16; ++c ) acc[ c ] = fma( u, cs[ l * 16 + c ], acc[ c ] );
       }
-      if ( side == 0 ) {
+      if ( side == 0 || kVcol ) {
          for ( int c = 0; c < nc; ++c ) out[ (long)( c0 + c ) * R + row ] = (E)acc[ c ];
       } else {
          for ( int c = 0; c < nc; ++c ) out[ row * r + c0 + c ] = (E)acc[ c ];
@@ -1300,6 +1302,319 @@ unsigned nblocks( long count, int threads ) { return (unsigned)( ( count + threa
 
 }  // namespace
 
+// ------------------------------------------------------------------ the hierarchical preconditioner
+//
+// The reference preconditions GMRES with a coarse H-LU: the H-matrix copied at eps_pre with the
+// near field compressed too, factored in place by the recursive H-LU (hmatrix/src/api.cpp:502-548,
+// hmatrix/src/hmatrix.cpp:516-1038) and applied once per step (lu_solve, :1126-1130).  That
+// recursion runs down the diagonal one leaf at a time (4,096 leaves in sequence at 245,760
+// unknowns), which on a GPU is a chain of tens of thousands of small dependent launches.  Here
+// the factorization is hierarchical in the other direction: every level of the cluster tree is
+// one batched step.  With the diagonal leaf blocks D (GR plus the deflation, exact) and, for every
+// cluster tau with sons a, b, the two blocks between the sons compressed at eps_pre,
+//     A(a, b) ~ U_ab V_ab^T,  A(b, a) ~ U_ba V_ba^T   (plus the deflation's rank-ndef terms),
+// the approximation M of A factors as M = D (I + W_1 Z_1^T) ... (I + W_{d} Z_{d}^T), one factor per
+// level, and M^-1 b is: the leaf inverses, then from the finest level to the root, at every
+// cluster of the level at once,
+//     y(tau) -= W^(tau) K_tau^-1 Z_tau^T y(tau),   K_tau = I + Z_tau^T W^(tau)   (2r x 2r),
+// W^ being the bases after the finer levels' inverses (Sherman-Morrison-Woodbury, applied to the
+// bases once during the factorization).  In the compact layout used here, level l keeps one
+// n3 x r_l column block of bases Wc (U_ab on the rows of a, U_ba on the rows of b) and one of Zc
+// (V_ba on the rows of a, V_ab on the rows of b); then Z_tau^T y = (Zc(b)^T y(b), Zc(a)^T y(a)) and
+// K_tau = [[I, Zc(b)^T W^c(b)], [Zc(a)^T W^c(a), I]].  The pieces are batched kernels: products
+// Zc(c)^T B over the rows of every son c of a level (k_hd_segtn, k_hd_segsum, fixed summation
+// order), the small systems K_tau (k_hd_kinv: through their r x r Schur complement, inverted by
+// Gauss-Jordan with partial pivoting in LDS, its pivots monitored as the reference's getrf_nopiv
+// monitors its own), u = K^-1 (..) (k_hd_node), and y(c) -= W^c(c) u (k_hd_update).
+
+constexpr int kHdRows = 1024;  // rows per chunk of k_hd_segtn
+constexpr int kHdCols = 32;    // columns per tile of k_hd_segtn and k_hd_update
+constexpr int kHdMaxR = 128;   // columns of a level's bases at most (compressed rank + deflation terms)
+constexpr int kHdUpd = 256;    // rows per workgroup of k_hd_update
+constexpr int kHdTile = 256;   // columns per pass of a level's correction
+
+// Partial products Zc(rows, 0:r)^T B(rows, j0 : j0 + kHdCols) over chunk k of kHdRows rows of son
+// ci of a level (heap cluster first + ci): grid.x = nchild * nchunk, grid.y = column tiles.
+// part[(x * q + j) * r + i] for j < q (q columns of B in all).  Rows are staged 32 at a time.
+__global__ void __launch_bounds__( 256 ) k_hd_segtn( int first, int nchunk, const int* __restrict__ cl_r0,
+                                                     const int* __restrict__ cl_nr, const double* __restrict__ Zc, long ldz,
+                                                     int r, const double* __restrict__ B, long ldb, int q,
+                                                     double* __restrict__ part ) {
+   constexpr int kOut = ( kHdMaxR * kHdCols + 255 ) / 256;
+   __shared__ double zs[ kHdMaxR * 32 ];
+   __shared__ double bs[ kHdCols * 32 ];
+   const int tid = threadIdx.x;
+   const int ci = blockIdx.x / nchunk, k = blockIdx.x % nchunk;
+   const int c = first + ci;
+   const long r0 = (long)cl_r0[ c ] + (long)k * kHdRows;
+   const int nrow = min( kHdRows, cl_nr[ c ] - k * kHdRows );
+   const int j0 = blockIdx.y * kHdCols, qt = min( kHdCols, q - j0 );
+   double acc[ kOut ];
+#pragma unroll
+   for ( int t = 0; t < kOut; ++t ) acc[ t ] = 0.0;
+   for ( int rr = 0; rr < nrow; rr += 32 ) {
+      const int nb = min( 32, nrow - rr );
+      __syncthreads();
+      for ( int e = tid; e < 32 * r; e += 256 ) {
+         const int row = e & 31, i = e >> 5;
+         zs[ i * 32 + row ] = row < nb ? Zc[ (long)i * ldz + r0 + rr + row ] : 0.0;
+      }
+      for ( int e = tid; e < 32 * kHdCols; e += 256 ) {
+         const int row = e & 31, j = e >> 5;
+         bs[ j * 32 + row ] = ( row < nb && j < qt ) ? B[ (long)( j0 + j ) * ldb + r0 + rr + row ] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for ( int t = 0; t < kOut; ++t ) {
+         const int o = tid + 256 * t;
+         if ( o >= r * qt ) break;
+         const int i = o % r, j = o / r;
+         double s = acc[ t ];
+         for ( int row = 0; row < 32; ++row ) s += zs[ i * 32 + row ] * bs[ j * 32 + row ];
+         acc[ t ] = s;
+      }
+   }
+#pragma unroll
+   for ( int t = 0; t < kOut; ++t ) {
+      const int o = tid + 256 * t;
+      if ( o >= r * qt ) break;
+      const int i = o % r, j = o / r;
+      part[ ( (long)blockIdx.x * q + j0 + j ) * r + i ] = acc[ t ];
+   }
+}
+
+// S[ci] (r x q, column-major) = the son's chunk partials added in order.
+__global__ void k_hd_segsum( int nchild, int nchunk, long rq, const double* __restrict__ part, double* __restrict__ S ) {
+   const long total = (long)nchild * rq;
+   for ( long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x ) {
+      const long ci = e / rq, o = e % rq;
+      double s = 0.0;
+      for ( int k = 0; k < nchunk; ++k ) s += part[ ( ci * nchunk + k ) * rq + o ];
+      S[ e ] = s;
+   }
+}
+
+// The cluster systems of a level: K = [[I, P_b], [P_a, I]] with P_a = S[2 nd], P_b = S[2 nd + 1]
+// (r x r, from k_hd_segsum with B = the level's bases).  K u = t is solved through the Schur
+// complement C = I - P_a P_b: u_2 = C^-1 (t_2 - P_a t_1), u_1 = t_1 - P_b u_2.  Per cluster nd
+// this stores P_a, P_b and C^-1 (3 r^2 at Kst + 3 nd r^2), C inverted by Gauss-Jordan with partial
+// pivoting in LDS; piv[2 nd], piv[2 nd + 1] = its smallest and largest |pivot|; a zero or
+// non-finite pivot sets *fail.
+__global__ void __launch_bounds__( 256 ) k_hd_kinv( int r, const double* __restrict__ S, double* __restrict__ Kst,
+                                                    double* __restrict__ piv, int* __restrict__ fail ) {
+   constexpr int LD = kHdMaxR + 1;
+   __shared__ double A[ kHdMaxR * LD ];
+   __shared__ int perm[ kHdMaxR ];
+   __shared__ double red[ 16 ];
+   __shared__ long ired[ 16 ];
+   const int nd = blockIdx.x, tid = threadIdx.x, n = r;
+   const long rr = (long)r * r;
+   const double* Pa = S + 2 * nd * rr;
+   const double* Pb = S + ( 2 * nd + 1 ) * rr;
+   double* out = Kst + 3 * nd * rr;
+   for ( long e = tid; e < rr; e += 256 ) out[ e ] = Pa[ e ], out[ rr + e ] = Pb[ e ];
+   // C = I - P_a P_b
+   for ( int e = tid; e < n * n; e += 256 ) {
+      const int i = e % n, j = e / n;
+      double s = 0.0;
+      for ( int k = 0; k < r; ++k ) s += Pa[ (long)k * r + i ] * Pb[ (long)j * r + k ];
+      A[ j * LD + i ] = ( i == j ? 1.0 : 0.0 ) - s;
+   }
+   if ( tid < n ) perm[ tid ] = tid;
+   __syncthreads();
+   double pmin = HUGE_VAL, pmax = 0.0;
+   for ( int k = 0; k < n; ++k ) {
+      double best = 0.0;
+      long bi = -1;
+      for ( int i = k + tid; i < n; i += 256 ) {
+         const double w = fabs( A[ k * LD + i ] );
+         if ( w > best ) best = w, bi = i;
+      }
+      if ( best == 0.0 ) bi = -1;
+      wg_argmax( best, bi, red, ired );
+      if ( bi < 0 || !isfinite( best ) ) {
+         if ( tid == 0 ) atomicOr( fail, 1 );
+         return;
+      }
+      pmin = fmin( pmin, best ), pmax = fmax( pmax, best );
+      const int pr = (int)bi;
+      if ( pr != k ) {
+         for ( int j = tid; j < n; j += 256 ) {
+            const double t = A[ j * LD + k ];
+            A[ j * LD + k ] = A[ j * LD + pr ];
+            A[ j * LD + pr ] = t;
+         }
+         if ( tid == 0 ) {
+            const int t = perm[ k ];
+            perm[ k ] = perm[ pr ];
+            perm[ pr ] = t;
+         }
+      }
+      __syncthreads();
+      const double d = 1.0 / A[ k * LD + k ];
+      __syncthreads();
+      for ( int j = tid; j < n; j += 256 ) A[ j * LD + k ] = ( j == k ? 1.0 : A[ j * LD + k ] ) * d;
+      __syncthreads();
+      for ( int e = tid; e < n * n; e += 256 ) {
+         const int i = e % n, j = e / n;
+         if ( i == k || j == k ) continue;
+         A[ j * LD + i ] -= A[ k * LD + i ] * A[ j * LD + k ];
+      }
+      __syncthreads();
+      for ( int i = tid; i < n; i += 256 )
+         if ( i != k ) A[ k * LD + i ] = -A[ k * LD + i ] * d;
+      __syncthreads();
+   }
+   for ( int e = tid; e < n * n; e += 256 ) {
+      const int i = e % n, j = e / n;
+      out[ 2 * rr + (long)perm[ j ] * n + i ] = A[ j * LD + i ];
+   }
+   if ( tid == 0 ) piv[ 2 * nd ] = pmin, piv[ 2 * nd + 1 ] = pmax;
+}
+
+// u[nd] (2r x q) = K_nd^-1 (t_1; t_2) with t_1 = S_b = S[2 nd + 1], t_2 = S_a = S[2 nd] (r x q
+// each): w = t_2 - P_a t_1, u_2 = C^-1 w, u_1 = t_1 - P_b u_2.  One workgroup per (cluster,
+// column tile of kHdCols); t_1, w and u_2 in LDS.
+__global__ void __launch_bounds__( 256 ) k_hd_node( int r, int q, const double* __restrict__ Kst, const double* __restrict__ S,
+                                                    double* __restrict__ u ) {
+   __shared__ double t1[ kHdMaxR * kHdCols ];
+   __shared__ double w[ kHdMaxR * kHdCols ];
+   __shared__ double u2[ kHdMaxR * kHdCols ];
+   const int nd = blockIdx.x, tid = threadIdx.x;
+   const int j0 = blockIdx.y * kHdCols, qt = min( kHdCols, q - j0 );
+   const long rr = (long)r * r, n2 = 2L * r;
+   const double* Pa = Kst + 3 * nd * rr;
+   const double* Pb = Pa + rr;
+   const double* Ci = Pa + 2 * rr;
+   const double* Sb = S + ( 2 * nd + 1 ) * (long)r * q;
+   const double* Sa = S + ( 2 * nd ) * (long)r * q;
+   double* un = u + (long)nd * n2 * q;
+   for ( int e = tid; e < r * qt; e += 256 ) {
+      const int i = e % r, j = e / r;
+      t1[ j * kHdMaxR + i ] = Sb[ (long)( j0 + j ) * r + i ];
+   }
+   __syncthreads();
+   for ( int e = tid; e < r * qt; e += 256 ) {  // w = t_2 - P_a t_1
+      const int i = e % r, j = e / r;
+      double s = Sa[ (long)( j0 + j ) * r + i ];
+      for ( int k = 0; k < r; ++k ) s -= Pa[ (long)k * r + i ] * t1[ j * kHdMaxR + k ];
+      w[ j * kHdMaxR + i ] = s;
+   }
+   __syncthreads();
+   for ( int e = tid; e < r * qt; e += 256 ) {  // u_2 = C^-1 w
+      const int i = e % r, j = e / r;
+      double s = 0.0;
+      for ( int k = 0; k < r; ++k ) s += Ci[ (long)k * r + i ] * w[ j * kHdMaxR + k ];
+      u2[ j * kHdMaxR + i ] = s;
+      un[ (long)( j0 + j ) * n2 + r + i ] = s;
+   }
+   __syncthreads();
+   for ( int e = tid; e < r * qt; e += 256 ) {  // u_1 = t_1 - P_b u_2
+      const int i = e % r, j = e / r;
+      double s = t1[ j * kHdMaxR + i ];
+      for ( int k = 0; k < r; ++k ) s -= Pb[ (long)k * r + i ] * u2[ j * kHdMaxR + k ];
+      un[ (long)( j0 + j ) * n2 + i ] = s;
+   }
+}
+
+// Y(rows of son ci, j0 : j0 + kHdCols) -= Wc(rows, 0:r) u[ci / 2](slot r : slot r + r, ...), slot
+// = ci & 1 (the first son takes the first r rows of u).  grid.x = nchild * nchunk (kHdUpd rows per
+// workgroup, one per thread), grid.y = column tiles.
+__global__ void __launch_bounds__( kHdUpd ) k_hd_update( int first, int nchunk, const int* __restrict__ cl_r0,
+                                                         const int* __restrict__ cl_nr, const double* __restrict__ Wc, long ldw,
+                                                         int r, const double* __restrict__ u, int q, double* __restrict__ Y,
+                                                         long ldy ) {
+   __shared__ double us[ kHdMaxR * kHdCols ];
+   const int ci = blockIdx.x / nchunk, k = blockIdx.x % nchunk;
+   const int c = first + ci, nd = ci >> 1, slot = ci & 1;
+   const int j0 = blockIdx.y * kHdCols, qt = min( kHdCols, q - j0 );
+   const long n2 = 2L * r;
+   for ( int e = threadIdx.x; e < r * kHdCols; e += kHdUpd ) {
+      const int kk = e % r, j = e / r;
+      us[ kk * kHdCols + j ] = j < qt ? u[ (long)nd * n2 * q + (long)( j0 + j ) * n2 + slot * r + kk ] : 0.0;
+   }
+   __syncthreads();
+   const int lr = k * kHdUpd + threadIdx.x;
+   if ( lr >= cl_nr[ c ] ) return;
+   const long row = (long)cl_r0[ c ] + lr;
+   double acc[ kHdCols ];
+#pragma unroll
+   for ( int j = 0; j < kHdCols; ++j ) acc[ j ] = 0.0;
+   for ( int kk = 0; kk < r; ++kk ) {
+      const double w = Wc[ (long)kk * ldw + row ];
+#pragma unroll
+      for ( int j = 0; j < kHdCols; ++j ) acc[ j ] += w * us[ kk * kHdCols + j ];
+   }
+   for ( int j = 0; j < qt; ++j ) Y[ (long)( j0 + j ) * ldy + row ] -= acc[ j ];
+}
+
+// One compressed block's factors into a level's bases: Wc(rows of t, col0 + 0:rank) = U and
+// Zc(rows of s, col0 + 0:rank) = V (both column-major, leading dimensions 3 cnt).
+struct HdScatter {
+   const double* U;
+   const double* V;
+   int rank, tr0, tnr, sr0, snr;
+};
+__global__ void k_hd_scatter( const HdScatter* __restrict__ jobs, double* __restrict__ Wc, double* __restrict__ Zc, long ld ) {
+   const HdScatter J = jobs[ blockIdx.x ];
+   for ( long e = threadIdx.x; e < (long)J.rank * J.tnr; e += blockDim.x ) {
+      const long l = e / J.tnr, i = e % J.tnr;
+      Wc[ l * ld + J.tr0 + i ] = J.U[ l * J.tnr + i ];
+   }
+   for ( long e = threadIdx.x; e < (long)J.rank * J.snr; e += blockDim.x ) {
+      const long l = e / J.snr, i = e % J.snr;
+      Zc[ l * ld + J.sr0 + i ] = J.V[ l * J.snr + i ];
+   }
+}
+
+// The deflation's terms in a level's bases, columns col0 + q (q < ndef): on every row i (tree
+// order) Wc = gdef_q xmu_i and Zc = xmu_i where row i is on surface q + 1 (all rows when ndef = 1),
+// which makes the block between the sons of every cluster carry sum_q gdef_q xmu_q(a) xmu_q(b)^T.
+__global__ void k_hd_deflation( int n3, int ndef, const double* __restrict__ xmu, const int* __restrict__ comp,
+                                const double* __restrict__ gdef, double* __restrict__ Wc, double* __restrict__ Zc, long ld,
+                                int col0 ) {
+   const long total = (long)n3 * ndef;
+   for ( long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x ) {
+      const int i = (int)( e % n3 ), q = (int)( e / n3 );
+      const bool on = ndef == 1 || comp[ i / 3 ] == q + 1;
+      Wc[ (long)( col0 + q ) * ld + i ] = on ? gdef[ q ] * xmu[ i ] : 0.0;
+      Zc[ (long)( col0 + q ) * ld + i ] = on ? xmu[ i ] : 0.0;
+   }
+}
+
+// The stacked factors of a block from its four sons' blocks (s00 s01 / s10 s11; the first
+// son's rows come first): U = [U00 U01 0 0; 0 0 U10 U11], V = [V00 0 V10 0; 0 V01 0 V11], column
+// blocks in the order 00, 01, 10, 11.  One workgroup per merged block.
+struct HdMerge {
+   const double* U[ 4 ];
+   const double* V[ 4 ];
+   int k[ 4 ];
+   int m0, m1, n0, n1;  // rows of the two row sons and of the two column sons (unknowns)
+   double* Uo;          // (m0 + m1) x sum k, column-major
+   double* Vo;          // (n0 + n1) x sum k
+};
+__global__ void k_hd_stack( const HdMerge* __restrict__ jobs ) {
+   const HdMerge J = jobs[ blockIdx.x ];
+   const int M = J.m0 + J.m1, N = J.n0 + J.n1;
+   int col = 0;
+   for ( int b = 0; b < 4; ++b ) {
+      const int ib = b >> 1, jb = b & 1;
+      const int mr = ib ? J.m1 : J.m0, moff = ib ? J.m0 : 0;
+      const int nr = jb ? J.n1 : J.n0, noff = jb ? J.n0 : 0;
+      for ( long e = threadIdx.x; e < (long)J.k[ b ] * M; e += blockDim.x ) {
+         const long l = e / M, i = e % M;
+         const long li = i - moff;
+         J.Uo[ ( col + l ) * M + i ] = ( li >= 0 && li < mr ) ? J.U[ b ][ l * mr + li ] : 0.0;
+      }
+      for ( long e = threadIdx.x; e < (long)J.k[ b ] * N; e += blockDim.x ) {
+         const long l = e / N, i = e % N;
+         const long li = i - noff;
+         J.Vo[ ( col + l ) * N + i ] = ( li >= 0 && li < nr ) ? J.V[ b ][ l * nr + li ] : 0.0;
+      }
+      col += J.k[ b ];
+   }
+}
+
 // ------------------------------------------------------------------ the H-matrix object
 
 struct HMatrix {
@@ -1331,6 +1646,20 @@ struct HMatrix {
    // deflation sums and Krylov work space
    DevBuf<double> defl_s, defl_part;
    DevBuf<double> Vb, Wk, Zk, Xk, Bk, Rk, part, hbuf;
+   // the hierarchical preconditioner: level l (clusters with sons, 0 .. hd_depth - 1) has the
+   // bases hd_W, hd_Z columns hd_off[l] .. hd_off[l] + hd_r[l] (hd_W holds W^ once factored) and
+   // its clusters' K^-1 at hd_K + hd_koff[l]
+   bool hd_on = false;
+   int hd_lmin = 0;  // the coarsest level applied (the levels above are left out)
+   KernelData K {};  // the kernel's tree-ordered data (for rebuilding the preconditioner)
+   int hd_depth = 0;
+   double hd_eps = 0.0, hd_piv_ratio = 0.0;
+   std::vector<int> hd_r;
+   std::vector<long> hd_off, hd_koff;
+   long hd_cols = 0;
+   DevBuf<double> hd_W, hd_Z, hd_K, hd_part, hd_S, hd_u;
+   DevBuf<int> cl_r0, cl_nr;
+   size_t hd_part_n = 0, hd_s_n = 0;
 
    void* alloc( size_t bytes ) {
       pools.emplace_back();
@@ -1382,6 +1711,45 @@ struct HMatrix {
    void precond( hipStream_t st, int p, const double* V, double* Z ) {
       hipLaunchKernelGGL( k_hm_precond, dim3( nleaves ), dim3( kRowsWg ), 0, st, leaf_r0.p, leaf_nr.p, pinv_off.p, pinv.p, p, V,
                           (long)n3, Z, (long)n3 );
+      if ( hd_on )
+         for ( int l = hd_depth - 1; l >= hd_lmin; --l ) hd_level( st, l, Z, (long)n3, p );
+   }
+
+   // Rows of the sons of level l's clusters: heap clusters first .. first + 2^(l+1) - 1.
+   int hd_first( int l ) const { return ( 2 << l ) - 1; }
+   int hd_chunks( int l, int rows ) const {
+      int mx = 0;
+      for ( int c = hd_first( l ); c < hd_first( l ) + ( 2 << l ); ++c ) mx = std::max( mx, 3 * T.cnt[ c ] );
+      return std::max( 1, ( mx + rows - 1 ) / rows );
+   }
+
+   // S = Zc_l(son)^T B(son) for every son of level l (r_l x q each, B's columns 0 .. q - 1).
+   void hd_products( hipStream_t st, int l, const double* B, long ldb, int q ) {
+      const int r = hd_r[ l ], nchild = 2 << l, nch = hd_chunks( l, kHdRows );
+      const double* Zc = hd_Z.p + hd_off[ l ] * (long)n3;
+      if ( (size_t)nchild * nch * r * q > hd_part_n || (size_t)nchild * r * q > hd_s_n )
+         throw std::runtime_error( "hierarchical preconditioner: work space" );
+      hipLaunchKernelGGL( k_hd_segtn, dim3( nchild * nch, ( q + kHdCols - 1 ) / kHdCols ), dim3( 256 ), 0, st, hd_first( l ), nch,
+                          cl_r0.p, cl_nr.p, Zc, (long)n3, r, B, ldb, q, hd_part.p );
+      const long rq = (long)r * q;
+      hipLaunchKernelGGL( k_hd_segsum, dim3( nblocks( nchild * rq, 256 ) ), dim3( 256 ), 0, st, nchild, nch, rq, hd_part.p, hd_S.p );
+   }
+
+   // B(:, 0 .. q-1) <- the correction of level l applied: B(tau) -= W^(tau) K_tau^-1 Z_tau^T B(tau),
+   // columns in tiles of kHdTile.
+   void hd_level( hipStream_t st, int l, double* B, long ldb, int q ) {
+      const int r = hd_r[ l ], nchild = 2 << l, nnode = 1 << l;
+      if ( r == 0 ) return;
+      const double* Wc = hd_W.p + hd_off[ l ] * (long)n3;
+      const int nchu = hd_chunks( l, kHdUpd );
+      for ( int j0 = 0; j0 < q; j0 += kHdTile ) {
+         const int qt = std::min( kHdTile, q - j0 );
+         hd_products( st, l, B + (long)j0 * ldb, ldb, qt );
+         hipLaunchKernelGGL( k_hd_node, dim3( nnode, ( qt + kHdCols - 1 ) / kHdCols ), dim3( 256 ), 0, st, r, qt,
+                             hd_K.p + hd_koff[ l ], hd_S.p, hd_u.p );
+         hipLaunchKernelGGL( k_hd_update, dim3( nchild * nchu, ( qt + kHdCols - 1 ) / kHdCols ), dim3( kHdUpd ), 0, st, hd_first( l ),
+                             nchu, cl_r0.p, cl_nr.p, Wc, (long)n3, r, hd_u.p, qt, B + (long)j0 * ldb, ldb );
+      }
    }
 
    // out[i p + q] = Vb_i(:, q) . W(:, q) for i < nv, copied to the host.
@@ -1395,6 +1763,408 @@ struct HMatrix {
 };
 
 void HMatrixDeleter::operator()( HMatrix* h ) const { delete h; }
+
+namespace {
+
+constexpr int kHdDirect = 5120;  // clusters of at most this many triangles: their blocks are approximated directly
+
+// One compressed block A(t, s) of the preconditioner: U (3 cnt(t) x rank) and V (3 cnt(s) x
+// rank), column-major, in the build's pools.
+struct HdRec {
+   int rank = 0;
+   const double* U = nullptr;
+   const double* V = nullptr;
+};
+
+// Truncates the candidates (U, V) of tj at eps (the reference's truncate, hmatrix/src/mat.cpp:
+// 129-160, by the recompression kernels), keeps at most cap[j] columns (the largest singular
+// values first), and stores the factors column-major in a new pool entry; out[j] receives them.
+void hd_truncate_store( hipStream_t st, std::vector<TruncJob>& tj, const std::vector<int>& cap, double eps,
+                        std::deque<DevBuf<char>>& pools, std::vector<HdRec*>& out ) {
+   const int nt = (int)tj.size();
+   if ( nt == 0 ) return;
+   // classes by kp (<= 32, 64, 96, 128), each with its own workgroup size and LDS
+   std::vector<int> order;
+   int ncls[ 4 ] = { 0, 0, 0, 0 };
+   for ( int cls = 0; cls < 4; ++cls )
+      for ( int j = 0; j < nt; ++j )
+         if ( tj[ j ].k >= 1 && ( tj[ j ].kp + 31 ) / 32 - 1 == cls ) order.push_back( j ), ++ncls[ cls ];
+   std::vector<TruncJob> jobs;
+   size_t scratch = 0;
+   for ( int j : order ) {
+      jobs.push_back( tj[ j ] );
+      jobs.back().G = reinterpret_cast<double*>( scratch );
+      scratch += 5 * (size_t)tj[ j ].kp * tj[ j ].kp + 2 * (size_t)tj[ j ].kp;
+   }
+   const int nj = (int)jobs.size();
+   std::vector<int> rank( nt, 0 );
+   if ( nj > 0 ) {
+      DevBuf<double> ws;
+      double* base = ws.ensure( scratch );
+      for ( TruncJob& t : jobs ) {
+         const size_t off = reinterpret_cast<size_t>( t.G ), kk = (size_t)t.kp * t.kp;
+         t.G = base + off;
+         t.Z = t.G + 2 * kk;
+         t.C = t.Z + kk;
+         t.d = t.C + 2 * kk;
+      }
+      DevBuf<TruncJob> jd;
+      DevBuf<int> rd, fd;
+      jd.ensure( nj );
+      rd.ensure( nj );
+      fd.ensure( nj );
+      BH_HIP( hipMemcpyAsync( jd.p, jobs.data(), sizeof( TruncJob ) * nj, hipMemcpyHostToDevice, st ) );
+      const int o1 = ncls[ 0 ], o2 = o1 + ncls[ 1 ], o3 = o2 + ncls[ 2 ];
+      if ( ncls[ 0 ] )
+         hipLaunchKernelGGL( ( k_hm_trunc<128, 32> ), dim3( ncls[ 0 ] ), dim3( 128 ), 0, st, jd.p, eps, rd.p, fd.p );
+      if ( ncls[ 1 ] )
+         hipLaunchKernelGGL( ( k_hm_trunc<256, 64> ), dim3( ncls[ 1 ] ), dim3( 256 ), 0, st, jd.p + o1, eps, rd.p + o1, fd.p + o1 );
+      if ( ncls[ 2 ] )
+         hipLaunchKernelGGL( ( k_hm_trunc<512, 96> ), dim3( ncls[ 2 ] ), dim3( 512 ), 0, st, jd.p + o2, eps, rd.p + o2, fd.p + o2 );
+      if ( ncls[ 3 ] )
+         hipLaunchKernelGGL( ( k_hm_trunc<512, 128> ), dim3( ncls[ 3 ] ), dim3( 512 ), 0, st, jd.p + o3, eps, rd.p + o3, fd.p + o3 );
+      std::vector<int> r( nj );
+      BH_HIP( hipMemcpyAsync( r.data(), rd.p, sizeof( int ) * nj, hipMemcpyDeviceToHost, st ) );
+      BH_HIP( hipStreamSynchronize( st ) );
+      for ( int q = 0; q < nj; ++q ) r[ q ] = std::min( r[ q ], cap[ order[ q ] ] );
+      size_t total = 0;
+      for ( int q = 0; q < nj; ++q ) total += (size_t)( jobs[ q ].M + jobs[ q ].N ) * r[ q ];
+      pools.emplace_back();
+      double* store = reinterpret_cast<double*>( pools.back().ensure( std::max<size_t>( total, 1 ) * sizeof( double ) ) );
+      std::vector<void*> uo( nj ), vo( nj );
+      std::vector<int4> tasks;
+      size_t at = 0;
+      for ( int q = 0; q < nj; ++q ) {
+         uo[ q ] = store + at;
+         vo[ q ] = store + at + (size_t)jobs[ q ].M * r[ q ];
+         at += (size_t)( jobs[ q ].M + jobs[ q ].N ) * r[ q ];
+         HdRec* o = out[ order[ q ] ];
+         o->rank = r[ q ];
+         o->U = static_cast<const double*>( uo[ q ] );
+         o->V = static_cast<const double*>( vo[ q ] );
+         if ( r[ q ] <= 0 ) continue;
+         for ( int side = 0; side < 2; ++side ) {
+            const int R = side ? jobs[ q ].N : jobs[ q ].M;
+            for ( int c = 0; c * kTrApply < R; ++c ) tasks.push_back( make_int4( q, side, c, 0 ) );
+         }
+      }
+      BH_HIP( hipMemcpyAsync( rd.p, r.data(), sizeof( int ) * nj, hipMemcpyHostToDevice, st ) );
+      if ( !tasks.empty() ) {
+         DevBuf<void*> ud, vd;
+         DevBuf<int4> td;
+         ud.ensure( nj );
+         vd.ensure( nj );
+         td.ensure( tasks.size() );
+         BH_HIP( hipMemcpyAsync( ud.p, uo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, st ) );
+         BH_HIP( hipMemcpyAsync( vd.p, vo.data(), sizeof( void* ) * nj, hipMemcpyHostToDevice, st ) );
+         BH_HIP( hipMemcpyAsync( td.p, tasks.data(), sizeof( int4 ) * tasks.size(), hipMemcpyHostToDevice, st ) );
+         hipLaunchKernelGGL( ( k_hm_trunc_apply<double, true> ), dim3( (unsigned)tasks.size() ), dim3( kTrApply ), 0, st, td.p, jd.p,
+                             rd.p, ud.p, vd.p );
+         BH_HIP( hipStreamSynchronize( st ) );
+      }
+      BH_HIP( hipStreamSynchronize( st ) );
+   }
+}
+
+}  // namespace
+
+// The hierarchical preconditioner of the built H-matrix (the section on it above): the blocks
+// between the sons of every cluster compressed at eps, directly by cross approximation where the
+// clusters have at most kHdDirect triangles, and above that by merging the four sons' blocks and
+// truncating again; then every level's bases with the deflation's terms, and their
+// factorization.  Returns false with a log line if a cluster's system is singular.
+bool hd_build( HMatrix& h, hipStream_t st, const KernelData& K, const HmOptions& opt, double eps, std::vector<std::string>* log ) {
+   auto say = [&]( const char* fmt, auto... a ) {
+      char buf[ 512 ];
+      std::snprintf( buf, sizeof buf, fmt, a... );
+      if ( log ) log->emplace_back( buf );
+   };
+   const double t0 = now();
+   const Tree& T = h.T;
+   const int D = T.depth, n3 = h.n3, ndef = h.ndef;
+   h.hd_on = false;
+   if ( D < 1 ) return false;
+   const int rfinal = kHdMaxR - ndef;  // a level's blocks: rank at most this (plus the deflation terms)
+   const int rmerge = kTrMaxK / 4;     // a block merged into a coarser one
+   int Ls = 0;
+   for ( ; Ls < D; ++Ls ) {
+      int mx = 0;
+      for ( int c = ( 1 << Ls ) - 1; c < ( 2 << Ls ) - 1; ++c ) mx = std::max( mx, T.cnt[ c ] );
+      if ( mx <= kHdDirect ) break;
+   }
+   const long NC = T.clusters();
+   std::vector<HdRec> recs;
+   std::vector<long> keys;
+   auto key = [&]( int t, int s ) { return (long)t * NC + s; };
+   auto sibling = [&]( int t, int s ) { return t != s && ( t - 1 ) / 2 == ( s - 1 ) / 2; };
+   // every block the build makes, in a sorted table (its key -> its record)
+   std::vector<std::pair<int, int>> pairs;
+   if ( Ls >= 1 )
+      for ( int t = ( 1 << Ls ) - 1; t < ( 2 << Ls ) - 1; ++t )
+         for ( int s = ( 1 << Ls ) - 1; s < ( 2 << Ls ) - 1; ++s )
+            if ( t != s ) pairs.emplace_back( t, s );
+   for ( int m = std::max( Ls, 0 ) + 1; m <= D; ++m )
+      for ( int c = ( 1 << ( m - 1 ) ) - 1; c < ( 1 << m ) - 1; ++c ) pairs.emplace_back( 2 * c + 1, 2 * c + 2 ), pairs.emplace_back( 2 * c + 2, 2 * c + 1 );
+   const size_t ndirect = pairs.size();
+   for ( int m = Ls - 1; m >= 1; --m )
+      for ( int t = ( 1 << m ) - 1; t < ( 2 << m ) - 1; ++t )
+         for ( int s = ( 1 << m ) - 1; s < ( 2 << m ) - 1; ++s )
+            if ( t != s ) pairs.emplace_back( t, s );
+   std::vector<size_t> idx( pairs.size() );
+   std::iota( idx.begin(), idx.end(), 0 );
+   std::sort( idx.begin(), idx.end(), [&]( size_t a, size_t b ) { return key( pairs[ a ].first, pairs[ a ].second ) < key( pairs[ b ].first, pairs[ b ].second ); } );
+   keys.resize( pairs.size() );
+   recs.resize( pairs.size() );
+   std::vector<size_t> slot( pairs.size() );
+   for ( size_t q = 0; q < idx.size(); ++q ) keys[ q ] = key( pairs[ idx[ q ] ].first, pairs[ idx[ q ] ].second ), slot[ idx[ q ] ] = q;
+   auto rec = [&]( int t, int s ) -> HdRec& {
+      const auto it = std::lower_bound( keys.begin(), keys.end(), key( t, s ) );
+      if ( it == keys.end() || *it != key( t, s ) ) throw std::runtime_error( "hierarchical preconditioner: missing block" );
+      return recs[ it - keys.begin() ];
+   };
+   std::deque<DevBuf<char>> pools;
+   long n_capped = 0;
+   // 1. cross approximation at aca_factor * eps of the direct blocks, largest first, in batches
+   //    within the work-space budget; each batch truncated to eps and stored
+   {
+      std::vector<size_t> dj( ndirect );
+      std::iota( dj.begin(), dj.end(), 0 );
+      std::stable_sort( dj.begin(), dj.end(), [&]( size_t a, size_t b ) {
+         return (long)T.cnt[ pairs[ a ].first ] + T.cnt[ pairs[ a ].second ] > (long)T.cnt[ pairs[ b ].first ] + T.cnt[ pairs[ b ].second ];
+      } );
+      auto kcap_of = [&]( size_t p ) {
+         return std::max( 3, std::min( kTrMaxK, 3 * std::min( T.cnt[ pairs[ p ].first ], T.cnt[ pairs[ p ].second ] ) / 2 ) );
+      };
+      auto need_of = [&]( size_t p ) {
+         const long m = T.cnt[ pairs[ p ].first ], nc = T.cnt[ pairs[ p ].second ];
+         return ( sizeof( double ) * ( (size_t)( 3 * m + 3 * nc ) * kcap_of( p ) + 9 * nc + 3 * m + 3 * nc ) + sizeof( int ) * ( m + 2 ) +
+                  255 ) / 256 * 256;
+      };
+      DevBuf<unsigned long long> calls;
+      calls.ensure( 1 );
+      BH_HIP( hipMemsetAsync( calls.p, 0, sizeof( unsigned long long ), st ) );
+      size_t pos = 0;
+      while ( pos < dj.size() ) {
+         std::vector<size_t> batch;
+         size_t bytes = 0;
+         while ( pos < dj.size() && ( batch.empty() || bytes + need_of( dj[ pos ] ) <= (size_t)opt.ws_bytes ) )
+            bytes += need_of( dj[ pos ] ), batch.push_back( dj[ pos++ ] );
+         DevBuf<char> ws;
+         char* base = ws.ensure( bytes );
+         std::vector<AcaJob> jobs;
+         size_t at = 0;
+         for ( size_t p : batch ) {
+            AcaJob j {};
+            j.rb = T.beg[ pairs[ p ].first ];
+            j.m = T.cnt[ pairs[ p ].first ];
+            j.cb = T.beg[ pairs[ p ].second ];
+            j.nc = T.cnt[ pairs[ p ].second ];
+            j.kcap = kcap_of( p );
+            double* d = reinterpret_cast<double*>( base + at );
+            at += need_of( p );
+            const long M = 3L * j.m, N = 3L * j.nc;
+            j.U = d;
+            j.V = d + M * j.kcap;
+            j.R = j.V + N * j.kcap;
+            j.u = j.R + 3 * N;
+            j.v = j.u + M;
+            j.used = reinterpret_cast<int*>( j.v + N );
+            jobs.push_back( j );
+         }
+         const int nj = (int)jobs.size();
+         DevBuf<AcaJob> jd;
+         DevBuf<int> rd;
+         jd.ensure( nj );
+         rd.ensure( nj );
+         BH_HIP( hipMemcpyAsync( jd.p, jobs.data(), sizeof( AcaJob ) * nj, hipMemcpyHostToDevice, st ) );
+         int nbig = 0;
+         while ( nbig < nj && 3L * ( jobs[ nbig ].m + jobs[ nbig ].nc ) >= kAcaBig ) ++nbig;
+         int nsm = nbig;
+         while ( nsm < nj && 3L * ( jobs[ nsm ].m + jobs[ nsm ].nc ) >= kAcaSmall ) ++nsm;
+         const double eaca = opt.aca_factor * eps;
+         if ( nbig ) hipLaunchKernelGGL( k_hm_aca<512>, dim3( nbig ), dim3( 512 ), 0, st, jd.p, K, eaca, rd.p, calls.p );
+         if ( nsm > nbig )
+            hipLaunchKernelGGL( k_hm_aca<kWg>, dim3( nsm - nbig ), dim3( kWg ), 0, st, jd.p + nbig, K, eaca, rd.p + nbig, calls.p );
+         if ( nj > nsm ) hipLaunchKernelGGL( k_hm_aca<64>, dim3( nj - nsm ), dim3( 64 ), 0, st, jd.p + nsm, K, eaca, rd.p + nsm, calls.p );
+         std::vector<int> rk( nj );
+         BH_HIP( hipMemcpyAsync( rk.data(), rd.p, sizeof( int ) * nj, hipMemcpyDeviceToHost, st ) );
+         BH_HIP( hipStreamSynchronize( st ) );
+         std::vector<TruncJob> tj( nj );
+         std::vector<int> cap( nj );
+         std::vector<HdRec*> out( nj );
+         for ( int q = 0; q < nj; ++q ) {
+            const auto& pr = pairs[ batch[ q ] ];
+            int k = rk[ q ];
+            if ( k < 0 ) k = jobs[ q ].kcap, ++n_capped;  // the cap reached: its terms so far
+            TruncJob t {};
+            t.U = jobs[ q ].U;
+            t.V = jobs[ q ].V;
+            t.M = 3 * jobs[ q ].m;
+            t.N = 3 * jobs[ q ].nc;
+            t.k = k;
+            t.kp = ( k + 3 ) / 4 * 4;
+            tj[ q ] = t;
+            cap[ q ] = sibling( pr.first, pr.second ) ? rfinal : rmerge;
+            out[ q ] = &recs[ slot[ batch[ q ] ] ];
+         }
+         hd_truncate_store( st, tj, cap, eps, pools, out );
+      }
+   }
+   const double t_aca = now() - t0;
+   // 2. the levels above: each block from its four sons' blocks, stacked and truncated again
+   for ( int m = Ls - 1; m >= 1; --m ) {
+      std::vector<std::pair<int, int>> lv;
+      for ( int t = ( 1 << m ) - 1; t < ( 2 << m ) - 1; ++t )
+         for ( int s = ( 1 << m ) - 1; s < ( 2 << m ) - 1; ++s )
+            if ( t != s ) lv.emplace_back( t, s );
+      std::vector<HdMerge> mj;
+      size_t total = 0;
+      for ( const auto& ts : lv ) {
+         HdMerge g {};
+         const int t = ts.first, s = ts.second;
+         for ( int b = 0; b < 4; ++b ) {
+            const HdRec& r = rec( 2 * t + 1 + ( b >> 1 ), 2 * s + 1 + ( b & 1 ) );
+            g.U[ b ] = r.U, g.V[ b ] = r.V, g.k[ b ] = r.rank;
+         }
+         g.m0 = 3 * T.cnt[ 2 * t + 1 ], g.m1 = 3 * T.cnt[ 2 * t + 2 ];
+         g.n0 = 3 * T.cnt[ 2 * s + 1 ], g.n1 = 3 * T.cnt[ 2 * s + 2 ];
+         const int kt = g.k[ 0 ] + g.k[ 1 ] + g.k[ 2 ] + g.k[ 3 ];
+         total += (size_t)( g.m0 + g.m1 + g.n0 + g.n1 ) * kt;
+         mj.push_back( g );
+      }
+      DevBuf<double> stk;
+      double* base = stk.ensure( std::max<size_t>( total, 1 ) );
+      size_t at = 0;
+      std::vector<TruncJob> tj( mj.size() );
+      std::vector<int> cap( mj.size() );
+      std::vector<HdRec*> out( mj.size() );
+      for ( size_t q = 0; q < mj.size(); ++q ) {
+         HdMerge& g = mj[ q ];
+         const int kt = g.k[ 0 ] + g.k[ 1 ] + g.k[ 2 ] + g.k[ 3 ];
+         g.Uo = base + at;
+         at += (size_t)( g.m0 + g.m1 ) * kt;
+         g.Vo = base + at;
+         at += (size_t)( g.n0 + g.n1 ) * kt;
+         TruncJob t {};
+         t.U = g.Uo;
+         t.V = g.Vo;
+         t.M = g.m0 + g.m1;
+         t.N = g.n0 + g.n1;
+         t.k = kt;
+         t.kp = ( kt + 3 ) / 4 * 4;
+         tj[ q ] = t;
+         cap[ q ] = sibling( lv[ q ].first, lv[ q ].second ) ? rfinal : rmerge;
+         out[ q ] = &rec( lv[ q ].first, lv[ q ].second );
+      }
+      DevBuf<HdMerge> md;
+      md.ensure( mj.size() );
+      BH_HIP( hipMemcpyAsync( md.p, mj.data(), sizeof( HdMerge ) * mj.size(), hipMemcpyHostToDevice, st ) );
+      hipLaunchKernelGGL( k_hd_stack, dim3( (unsigned)mj.size() ), dim3( 256 ), 0, st, md.p );
+      hd_truncate_store( st, tj, cap, eps, pools, out );
+   }
+   // 3. the bases of every level: the blocks between its clusters' sons, and the deflation
+   h.hd_depth = D;
+   h.hd_r.assign( D, 0 );
+   h.hd_off.assign( D + 1, 0 );
+   h.hd_koff.assign( D + 1, 0 );
+   for ( int l = 0; l < D; ++l ) {
+      int r = 0;
+      for ( int c = ( 1 << l ) - 1; c < ( 2 << l ) - 1; ++c )
+         r = std::max( { r, rec( 2 * c + 1, 2 * c + 2 ).rank, rec( 2 * c + 2, 2 * c + 1 ).rank } );
+      h.hd_r[ l ] = r + ndef;
+      h.hd_off[ l + 1 ] = h.hd_off[ l ] + h.hd_r[ l ];
+      h.hd_koff[ l + 1 ] = h.hd_koff[ l ] + ( 1L << l ) * 3L * h.hd_r[ l ] * h.hd_r[ l ];
+   }
+   h.hd_cols = h.hd_off[ D ];
+   h.hd_W.ensure( (size_t)n3 * h.hd_cols );
+   h.hd_Z.ensure( (size_t)n3 * h.hd_cols );
+   h.hd_K.ensure( std::max<long>( h.hd_koff[ D ], 1 ) );
+   BH_HIP( hipMemsetAsync( h.hd_W.p, 0, sizeof( double ) * n3 * h.hd_cols, st ) );
+   BH_HIP( hipMemsetAsync( h.hd_Z.p, 0, sizeof( double ) * n3 * h.hd_cols, st ) );
+   {
+      std::vector<int> r0( NC ), nr( NC );
+      for ( long c = 0; c < NC; ++c ) r0[ c ] = 3 * T.beg[ c ], nr[ c ] = 3 * T.cnt[ c ];
+      h.cl_r0.ensure( NC );
+      h.cl_nr.ensure( NC );
+      BH_HIP( hipMemcpyAsync( h.cl_r0.p, r0.data(), sizeof( int ) * NC, hipMemcpyHostToDevice, st ) );
+      BH_HIP( hipMemcpyAsync( h.cl_nr.p, nr.data(), sizeof( int ) * NC, hipMemcpyHostToDevice, st ) );
+      DevBuf<HdScatter> sd;
+      for ( int l = 0; l < D; ++l ) {
+         std::vector<HdScatter> sj;
+         for ( int c = ( 1 << l ) - 1; c < ( 2 << l ) - 1; ++c )
+            for ( int side = 0; side < 2; ++side ) {
+               const int t = 2 * c + 1 + side, s = 2 * c + 2 - side;
+               const HdRec& r = rec( t, s );
+               if ( r.rank > 0 ) sj.push_back( HdScatter { r.U, r.V, r.rank, 3 * T.beg[ t ], 3 * T.cnt[ t ], 3 * T.beg[ s ], 3 * T.cnt[ s ] } );
+            }
+         double* Wc = h.hd_W.p + h.hd_off[ l ] * (long)n3;
+         double* Zc = h.hd_Z.p + h.hd_off[ l ] * (long)n3;
+         if ( !sj.empty() ) {
+            sd.ensure( sj.size() );
+            BH_HIP( hipMemcpyAsync( sd.p, sj.data(), sizeof( HdScatter ) * sj.size(), hipMemcpyHostToDevice, st ) );
+            hipLaunchKernelGGL( k_hd_scatter, dim3( (unsigned)sj.size() ), dim3( 256 ), 0, st, sd.p, Wc, Zc, (long)n3 );
+            BH_HIP( hipStreamSynchronize( st ) );  // sd is reused by the next level
+         }
+         hipLaunchKernelGGL( k_hd_deflation, dim3( nblocks( (long)n3 * ndef, 256 ) ), dim3( 256 ), 0, st, n3, ndef, h.xmu_t.p,
+                             h.comp_t.p, h.gdef.p, Wc, Zc, (long)n3, h.hd_r[ l ] - ndef );
+      }
+   }
+   BH_HIP( hipStreamSynchronize( st ) );
+   pools.clear();  // the blocks are in the bases now
+   // work space of the level corrections (kHdTile columns at a time)
+   h.hd_part_n = h.hd_s_n = 1;
+   for ( int l = 0; l < D; ++l ) {
+      const size_t r = h.hd_r[ l ], nchild = (size_t)2 << l;
+      h.hd_part_n = std::max( h.hd_part_n, nchild * h.hd_chunks( l, kHdRows ) * r * kHdTile );
+      h.hd_s_n = std::max( h.hd_s_n, nchild * r * kHdTile );
+   }
+   h.hd_part.ensure( h.hd_part_n );
+   h.hd_S.ensure( h.hd_s_n );
+   h.hd_u.ensure( h.hd_s_n );
+   // 4. the factorization: the leaf inverses on every basis, then from the finest level up, each
+   //    level's cluster systems and its correction of the coarser levels' bases
+   for ( long c0 = 0; c0 < h.hd_cols; c0 += kMaxRhs ) {
+      double* B = h.hd_W.p + c0 * n3;
+      hipLaunchKernelGGL( k_hm_precond, dim3( h.nleaves ), dim3( kRowsWg ), 0, st, h.leaf_r0.p, h.leaf_nr.p, h.pinv_off.p, h.pinv.p,
+                          (int)std::min<long>( kMaxRhs, h.hd_cols - c0 ), B, (long)n3, B, (long)n3 );
+   }
+   DevBuf<double> piv;
+   DevBuf<int> fail;
+   piv.ensure( (size_t)2 << D );
+   fail.ensure( 1 );
+   BH_HIP( hipMemsetAsync( fail.p, 0, sizeof( int ), st ) );
+   long piv_at = 0;
+   for ( int l = D - 1; l >= 0; --l ) {
+      const int r = h.hd_r[ l ];
+      if ( r == 0 ) continue;
+      h.hd_products( st, l, h.hd_W.p + h.hd_off[ l ] * (long)n3, (long)n3, r );
+      hipLaunchKernelGGL( k_hd_kinv, dim3( 1 << l ), dim3( 256 ), 0, st, r, h.hd_S.p, h.hd_K.p + h.hd_koff[ l ], piv.p + piv_at,
+                          fail.p );
+      piv_at += 2L << l;
+      if ( h.hd_off[ l ] > 0 ) h.hd_level( st, l, h.hd_W.p, (long)n3, (int)h.hd_off[ l ] );
+   }
+   int hf = 0;
+   std::vector<double> pv( piv_at );
+   BH_HIP( hipMemcpyAsync( &hf, fail.p, sizeof hf, hipMemcpyDeviceToHost, st ) );
+   if ( piv_at ) BH_HIP( hipMemcpyAsync( pv.data(), piv.p, sizeof( double ) * piv_at, hipMemcpyDeviceToHost, st ) );
+   BH_HIP( hipStreamSynchronize( st ) );
+   if ( hf ) {
+      say( "preconditioner: the hierarchical factorization at eps %.0e broke down (a singular cluster system)", eps );
+      return false;
+   }
+   double pmin = HUGE_VAL, pmax = 0.0;
+   for ( long q = 0; q < piv_at; q += 2 ) pmin = std::min( pmin, pv[ q ] ), pmax = std::max( pmax, pv[ q + 1 ] );
+   h.hd_piv_ratio = pmax > 0 ? pmin / pmax : 0.0;
+   h.hd_eps = eps;
+   h.hd_on = true;
+   h.hd_lmin = std::min( D, opt.pre_skip );
+   int rmax = 0;
+   for ( int r : h.hd_r ) rmax = std::max( rmax, r );
+   say( "preconditioner: hierarchical factorization at eps %.0e over %d levels in %.3f s (approximations %.3f s, %ld at "
+        "their cap), level ranks up to %d (%ld basis columns, %.2f GB), smallest pivot ratio of the cluster systems %.2e",
+        eps, D, now() - t0, t_aca, n_capped, rmax, h.hd_cols, 2.0 * 8.0 * n3 * h.hd_cols / 1e9, h.hd_piv_ratio );
+   return true;
+}
 
 HmStats hm_plan( int n, const double* h_v, const double* h_xc, const HmOptions& opt ) {
    HmStats S;
@@ -1952,8 +2722,24 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
    BH_HIP( hipMemcpyAsync( &hf, fail.p, sizeof hf, hipMemcpyDeviceToHost, st ) );
    BH_HIP( hipStreamSynchronize( st ) );
    if ( hf ) throw std::runtime_error( "H-matrix: a diagonal leaf block is singular" );
+   h.K = K;
+   if ( opt.precond == 1 || opt.direct ) {
+      const double e = opt.direct ? opt.eps : opt.eps_pre;
+      if ( hd_build( h, st, K, opt, e, log ) ) {
+         S.eps_pre = h.hd_eps;
+         S.piv_ratio = h.hd_piv_ratio;
+         S.pre_levels = h.hd_depth;
+         for ( int r : h.hd_r ) S.pre_rank_max = std::max( S.pre_rank_max, r );
+         S.pre_bytes = 2.0 * 8.0 * h.n3 * (double)h.hd_cols;
+      } else if ( opt.direct ) {
+         throw std::runtime_error( "H-LU: the factorization broke down; without -hlu BEST solves by GMRES" );
+      } else {
+         say( "preconditioner: block Jacobi on the diagonal leaf blocks alone" );
+      }
+   }
    S.t_pre = now() - t0;
-   say( "preconditioner: block Jacobi on the %d diagonal leaf blocks, %.3f s", h.nleaves, S.t_pre );
+   say( "preconditioner: %s, %.3f s", h.hd_on ? "the hierarchical factorization on the block-Jacobi leaf inverses"
+                                               : "block Jacobi on the diagonal leaf blocks", S.t_pre );
    h.defl_s.ensure( (size_t)h.ndef * kMaxRhs );
    h.defl_part.ensure( (size_t)( ( h.n3 + kDotChunkDefl - 1 ) / kDotChunkDefl ) * h.ndef * kMaxRhs );
    h.ypart.ensure( (size_t)h.nleaves * kSplit * kRowsWg * kMaxRhs );
@@ -2016,9 +2802,12 @@ double hm_time_product( HMatrix& h, hipStream_t st, int p, int reps, double* byt
    return ms / reps;
 }
 
-void hm_solve( HMatrix& h, hipStream_t st, int p, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
+namespace {
+
+// Restarted GMRES (hmatrix/src/gmres.cpp:26-178) on the built H-matrix with its preconditioner;
+// returns false if it did not reach the tolerance within maxsteps (the solution so far is in d_x).
+bool hm_gmres( HMatrix& h, hipStream_t st, int p, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
                std::vector<std::string>* log ) {
-   if ( p < 1 || p > kMaxRhs ) throw std::runtime_error( "H-matrix solve: 1 to 16 right-hand sides per call" );
    const double t_start = now();
    const int n3 = h.n3, m = h.opt.restart;
    const double tol = h.opt.tol > 0 ? h.opt.tol : h.opt.aca_factor * h.opt.eps;
@@ -2168,11 +2957,121 @@ void hm_solve( HMatrix& h, hipStream_t st, int p, const double* d_b, long ldb, d
       stats->t_solve = el;
    }
    if ( !( worst <= tol ) ) {
-      char buf[ 160 ];
-      std::snprintf( buf, sizeof buf, "GMRES did not reach a relative residual of %.0e in %d steps (%.1e)", tol,
-                     h.opt.maxsteps, worst );
+      if ( log ) {
+         char buf[ 160 ];
+         std::snprintf( buf, sizeof buf, "GMRES did not reach a relative residual of %.0e in %d steps (%.1e)", tol,
+                        h.opt.maxsteps, worst );
+         log->emplace_back( buf );
+      }
+      return false;
+   }
+   return true;
+}
+
+// -hlu: x = M^-1 b with the hierarchical factorization at eps, refined against the H-matrix
+// (x += M^-1 (b - A x)) until the relative residual is below the tolerance; no Krylov method.
+// The reference's direct mode (hmatrix/src/api.cpp:461-495) solves with its H-LU factors once.
+constexpr int kDirectSteps = 40;
+void hm_direct( HMatrix& h, hipStream_t st, int p, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
+                std::vector<std::string>* log ) {
+   const double t_start = now();
+   const int n3 = h.n3;
+   const double tol = h.opt.tol > 0 ? h.opt.tol : h.opt.aca_factor * h.opt.eps;
+   const size_t nn = (size_t)n3 * p;
+   h.Wk.ensure( nn );
+   h.Zk.ensure( nn );
+   h.Xk.ensure( nn );
+   h.Bk.ensure( nn );
+   h.Rk.ensure( nn );
+   h.part.ensure( (size_t)( ( n3 + kDotChunk - 1 ) / kDotChunk ) * kMaxRhs );
+   h.hbuf.ensure( kMaxRhs );
+   const unsigned g = nblocks( nn, 256 );
+   hipLaunchKernelGGL( k_hm_reorder, dim3( g ), dim3( 256 ), 0, st, h.n, p, h.perm.p, 1, d_b, ldb, h.Bk.p, (long)n3 );
+   std::vector<double> bn( p ), rn( p );
+   h.dots( st, p, 1, h.Bk.p, 0, h.Bk.p, bn.data() );
+   h.precond( st, p, h.Bk.p, h.Xk.p );
+   int steps = 0;
+   double worst = 0.0;
+   for ( ;; ) {
+      h.apply( st, p, h.Xk.p, h.Wk.p );
+      hipLaunchKernelGGL( k_hm_axpy, dim3( g ), dim3( 256 ), 0, st, (long)nn, h.Bk.p, -1.0, h.Wk.p, h.Rk.p );
+      h.dots( st, p, 1, h.Rk.p, 0, h.Rk.p, rn.data() );
+      worst = 0.0;
+      for ( int q = 0; q < p; ++q ) {
+         const double r = bn[ q ] > 0 ? std::sqrt( rn[ q ] / bn[ q ] ) : 0.0;
+         worst = ( std::isnan( r ) || r > worst ) ? r : worst;
+      }
+      if ( worst <= tol || steps >= kDirectSteps || !std::isfinite( worst ) ) break;
+      h.precond( st, p, h.Rk.p, h.Zk.p );
+      hipLaunchKernelGGL( k_hm_axpy, dim3( g ), dim3( 256 ), 0, st, (long)nn, h.Xk.p, 1.0, h.Zk.p, h.Xk.p );
+      ++steps;
+   }
+   hipLaunchKernelGGL( k_hm_reorder, dim3( g ), dim3( 256 ), 0, st, h.n, p, h.perm.p, 0, h.Xk.p, (long)n3, d_x, ldx );
+   BH_HIP( hipStreamSynchronize( st ) );
+   const double el = now() - t_start;
+   if ( log ) {
+      char buf[ 256 ];
+      std::snprintf( buf, sizeof buf,
+                     "solve: %d right-hand sides with the hierarchical factorization at eps %.0e (-hlu), %d refinement steps, "
+                     "largest relative residual %.1e, %.3f s",
+                     p, h.hd_eps, steps, worst, el );
+      log->emplace_back( buf );
+   }
+   if ( stats ) {
+      stats->iters_min = stats->iters_max = steps;
+      stats->relres = worst;
+      stats->steps = steps;
+      stats->t_solve = el;
+   }
+   if ( !( worst <= tol ) ) {
+      char buf[ 200 ];
+      std::snprintf( buf, sizeof buf,
+                     "H-LU: the factorization at eps %.0e does not reach a relative residual of %.0e (%.1e after %d refinement "
+                     "steps); without -hlu BEST solves by GMRES",
+                     h.hd_eps, tol, worst, steps );
       throw std::runtime_error( buf );
    }
 }
+
+constexpr double kPreMin = 1e-3;  // the finest preconditioner the retries build (hmatrix/src/api.cpp:554-578)
+
+}  // namespace
+
+void hm_solve( HMatrix& h, hipStream_t st, int p, const double* d_b, long ldb, double* d_x, long ldx, HmStats* stats,
+               std::vector<std::string>* log ) {
+   if ( p < 1 || p > kMaxRhs ) throw std::runtime_error( "H-matrix solve: 1 to 16 right-hand sides per call" );
+   if ( h.opt.direct ) {
+      hm_direct( h, st, p, d_b, ldb, d_x, ldx, stats, log );
+      return;
+   }
+   // GMRES; if it does not converge, a stronger preconditioner is built and GMRES runs again: the
+   // hierarchical factorization at eps_pre after block Jacobi, then that factorization 10 times
+   // finer (down to 1e-3), as the reference's finer() does (hmatrix/src/api.cpp:554-578, :744-763)
+   for ( ;; ) {
+      if ( hm_gmres( h, st, p, d_b, ldb, d_x, ldx, stats, log ) ) return;
+      if ( !h.hd_on ) {
+         if ( log ) log->emplace_back( "GMRES preconditioner: block Jacobi was not enough; building the hierarchical "
+                                       "factorization" );
+         if ( !hd_build( h, st, h.K, h.opt, h.opt.eps_pre, log ) ) throw std::runtime_error( "GMRES preconditioner: broke down" );
+         if ( stats ) stats->eps_pre = h.hd_eps, stats->piv_ratio = h.hd_piv_ratio;
+         continue;
+      }
+      if ( h.hd_eps <= kPreMin * 1.001 ) {
+         char buf[ 160 ];
+         std::snprintf( buf, sizeof buf, "GMRES did not reach its relative residual in %d steps (%.1e)", h.opt.maxsteps,
+                        stats ? stats->relres : 0.0 );
+         throw std::runtime_error( buf );
+      }
+      const double next = h.hd_eps / 10;
+      if ( log ) {
+         char buf[ 96 ];
+         std::snprintf( buf, sizeof buf, "GMRES preconditioner: rebuilding it at eps %.0e", next );
+         log->emplace_back( buf );
+      }
+      if ( !hd_build( h, st, h.K, h.opt, next, log ) ) throw std::runtime_error( "GMRES preconditioner: broke down" );
+      if ( stats ) stats->eps_pre = h.hd_eps, stats->piv_ratio = h.hd_piv_ratio;
+   }
+}
+
 
 }  // namespace bhg

@@ -1188,6 +1188,10 @@ int bhg_set_hm_option( bhg_solver* s, const char* name, double value ) {
    else if ( k == "maxsteps" ) o.maxsteps = (int)value;
    else if ( k == "ws_gb" ) o.ws_bytes = value * 1e9;
    else if ( k == "trunc" ) o.trunc = value != 0.0 ? 1 : 0;
+   else if ( k == "precond" ) o.precond = value != 0.0 ? 1 : 0;
+   else if ( k == "eps_pre" && value > 0.0 ) o.eps_pre = value;
+   else if ( k == "direct" ) o.direct = value != 0.0 ? 1 : 0;
+   else if ( k == "pre_skip" ) o.pre_skip = std::max( 0, (int)value );
    else return 1;
    return 0;
 }
@@ -1219,6 +1223,11 @@ int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out ) {
    out->n_trunc_failed = h.n_trunc_failed;
    out->kmean_aca = h.kmean_aca;
    out->t_trunc = h.t_trunc;
+   out->eps_pre = h.eps_pre;
+   out->piv_ratio = h.piv_ratio;
+   out->pre_levels = h.pre_levels;
+   out->pre_rank_max = h.pre_rank_max;
+   out->pre_bytes = h.pre_bytes;
    return 0;
 }
 
@@ -1806,7 +1815,6 @@ double bhm_diag_mean( void ) {
 
 int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* gdef, const int* comp, const double* xmu,
                int single, int nrhs ) {
-   (void)eps_pre;
    (void)nrhs;
    if ( !g_solver ) {
       g_log.emplace_back( "error: bhm_build before bhm_create" );
@@ -1823,7 +1831,15 @@ int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* g
          s->hm_opt.eps = eps > 0.0 ? eps : 1e-4;
          s->hm_opt.single = single ? 1 : 0;
          s->hm_opt.tol = tol ? std::atof( tol ) : 0.0;
-         if ( direct ) s->say( "the direct H-LU (-hlu) is not on the GPU path: GMRES runs to the same tolerance instead" );
+         // BEST_HMATRIX_PRE_EPS overrides the preconditioner's accuracy (hmatrix/best_hmatrix.h:48-60)
+         const char* pre = std::getenv( "BEST_HMATRIX_PRE_EPS" );
+         s->hm_opt.eps_pre = pre ? std::atof( pre ) : ( eps_pre > 0.0 ? eps_pre : 0.1 );
+         s->hm_opt.direct = direct ? 1 : 0;
+         // BEST_HIP_PRECOND=hierarchical: GMRES preconditioned by the hierarchical factorization at
+         // eps_pre from the start (the reference's choice, hmatrix/best_hmatrix.h:48-60); by default
+         // block Jacobi, which is faster here, and the factorization only if GMRES stalls
+         const char* pc = std::getenv( "BEST_HIP_PRECOND" );
+         if ( pc ) s->hm_opt.precond = std::string( pc ) == "hierarchical" ? 1 : 0;
          s->ndef = std::max( ndef, 1 );
          s->gdef.ensure( s->ndef );
          BH_HIP( hipMemcpyAsync( s->gdef.p, gdef, s->ndef * sizeof( double ), hipMemcpyHostToDevice, s->st ) );

@@ -42,11 +42,15 @@ void bhm_entries( int i0, int m, int j0, int nc, double* a, int lda );
 /* hmatrix/best_hmatrix.h:46: the mean diagonal entry of GR in BEST's summation order. */
 double bhm_diag_mean( void );
 
-/* hmatrix/best_hmatrix.h:62-63.  Assembles GR + sum_k gdef[k] xmu_k xmu_k^T densely on the GPU
- * and factors it: single = 1 (or BEST_HIP_LU unset) FP32 LU on MFMA with FP64 iterative
- * refinement in bhm_solve; BEST_HIP_LU=fp64 and single = 0: FP64 LU.  eps, direct, eps_pre and
- * nrhs are accepted for the ABI; the dense solve is exact to rounding.  Returns nonzero (with a
- * message) when the matrix does not fit in device memory. */
+/* hmatrix/best_hmatrix.h:62-63.  Up to 30,000 unknowns (or with BEST_HIP_SOLVER=dense): assembles
+ * GR + sum_k gdef[k] xmu_k xmu_k^T densely on the GPU and factors it: single = 1 (or BEST_HIP_LU
+ * unset) FP32 LU on MFMA with FP64 iterative refinement in bhm_solve; BEST_HIP_LU=fp64 and single
+ * = 0: FP64 LU; eps, direct and eps_pre do not apply (the dense solve is exact to rounding).  Above
+ * that (or with BEST_HIP_SOLVER=hmatrix): the H-matrix at eps (FP32 storage with single = 1), and
+ * with direct = 0 GMRES preconditioned by block Jacobi, then by the hierarchical factorization at
+ * eps_pre if GMRES stalls (BEST_HIP_PRECOND=hierarchical: from the start); with direct = 1 that
+ * factorization at eps solves directly.  Returns nonzero (with a message) when the problem does not
+ * fit in device memory. */
 int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* gdef, const int* comp, const double* xmu,
                int single, int nrhs );
 
@@ -188,6 +192,10 @@ typedef struct bhg_hm_stats {
    double relres, t_solve;
    long n_trunc_failed; /* approximations not recompressed (dependent columns or rank above 128) */
    double kmean_aca, t_trunc; /* mean rank before the recompression; its time (s, part of t_fill) */
+   double eps_pre;            /* accuracy of the hierarchical preconditioner (0: block Jacobi only) */
+   double piv_ratio;          /* its smallest pivot ratio (the cluster systems) */
+   int pre_levels, pre_rank_max; /* its levels and their largest basis width */
+   double pre_bytes;          /* device bytes of its bases */
 } bhg_hm_stats;
 
 /* method 0: dense LU (default); 1: H-matrix + GMRES (hmatrix/src/api.cpp:585-735) for the next
@@ -201,7 +209,10 @@ int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out );
 /* The H-matrix settings that are not BEST options (hmatrix/src/api.cpp:37-41, hmatrix.h:60-66):
  * "eta" (admissibility, 2), "aca_factor" (0.1), "leaf" (triangles per leaf cluster, 32),
  * "dense_small" (admissible blocks of at most this many triangle pairs stay dense, 1024), "kcap"
- * (rank at which an approximation is abandoned for a dense block, 192), "restart" (GMRES restart
+ * (rank at which an approximation is abandoned for a dense block, 192), "precond" (0: block
+ * Jacobi, then the hierarchical factorization at eps_pre if GMRES stalls; 1: the factorization from
+ * the start, hmatrix/src/api.cpp:502-578), "eps_pre" (its accuracy, 0.1), "direct" (1: -hlu, the
+ * factorization at eps refined against the H-matrix, hmatrix/src/api.cpp:461-495), "restart" (GMRES restart
  * length, 120: the reference restarts at 30, hmatrix/src/api.cpp:39, with its H-LU preconditioner;
  * with block Jacobi the longer basis saves iterations, 109 -> 78 on 245,760 unknowns),
  * "maxsteps" (500), "ws_gb" (device work space of the cross approximations, GB), "trunc" (1:

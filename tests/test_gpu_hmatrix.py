@@ -2,7 +2,7 @@
 the dense FP64 oracle and against the reference's own H-matrix library (oracle/_ref).
 
 Bars: the product (GR + deflation) x within 10 eps of the dense product; GMRES to its relative
-residual; D_t, D_r and [eta] within 1e-5 (eps = 1e-6) of the dense answers; the reference library
+residual (preconditioned by the hierarchical factorization at eps_pre, or block Jacobi); D_t, D_r and [eta] within 1e-5 (eps = 1e-6) of the dense answers; the reference library
 at the same eps agrees with the dense answers to the same order, so the two H-matrix answers agree
 to twice that."""
 import os
@@ -237,3 +237,110 @@ def test_c5_matches_reference_library(bh, solver):
     for key in ("dtt", "drr", "dtr"):
         assert errs[key] < 1e-7, key
     solver.set_method("dense")
+
+
+def test_hierarchical_preconditioner(bh, solver):
+    """GMRES preconditioned by the hierarchical factorization of the H-matrix at eps_pre = 0.1
+    (BEST's EPSPRE) on the 2,562-triangle example: fewer iterations than block Jacobi on the diagonal
+    leaves, and the same answers (within 1e-8 of the dense GPU solve; the H-matrix at eps 1e-6,
+    GMRES to 1e-10)."""
+    from best_hydro_amd import meshes as M
+    solver.set_mesh(M.read_best(os.path.join(GOLDEN, "2eu7_02562")))
+    d = solver.solve_mesh(ivisco=1, precision=64)
+    iters = {}
+    for pc in (0, 1):
+        solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+        solver.set_hm_option("precond", pc)
+        h = solver.solve_mesh(ivisco=1)
+        st = solver.hm_stats()
+        iters[pc] = st["iters_max"]
+        print(f"precond {pc}: {st['iters_min']}..{st['iters_max']} iterations, eps_pre {st['eps_pre']}, pivot ratio "
+              f"{st['piv_ratio']:.1e}, levels {st['pre_levels']}, widest basis {st['pre_rank_max']}", flush=True)
+        assert st["relres"] <= 1e-10
+        for key in ("dtrans", "dr", "chiprime"):
+            assert rel(h[key], d[key]) < 1e-8, (pc, key)
+        if pc:
+            assert st["eps_pre"] == 0.1 and st["piv_ratio"] > 0 and st["pre_levels"] == 7
+    assert iters[1] < 0.7 * iters[0]
+    solver.set_hm_option("precond", 0)
+    solver.set_method("dense")
+
+
+def test_preconditioner_rebuilt_finer(bh, solver):
+    """GMRES that does not converge within its step limit rebuilds its preconditioner ten times
+    finer and runs again (the reference's finer(), hmatrix/src/api.cpp:554-578): eps_pre 0.3 with
+    25 steps allowed."""
+    from best_hydro_amd import meshes as M
+    solver.set_mesh(M.read_best(os.path.join(GOLDEN, "2eu7_02562")))
+    solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+    solver.set_hm_option("precond", 1)
+    solver.set_hm_option("eps_pre", 0.3)
+    solver.set_hm_option("maxsteps", 25)
+    h = solver.solve_mesh(ivisco=0)
+    st = solver.hm_stats()
+    log = solver.log()
+    print("\n".join(l for l in log if "precond" in l or "GMRES" in l), flush=True)
+    assert any("rebuilding it at eps 3e-02" in l for l in log)
+    assert abs(st["eps_pre"] - 0.03) < 1e-12 and st["relres"] <= 1e-10
+    solver.set_hm_option("eps_pre", 0.1)
+    solver.set_hm_option("maxsteps", 500)
+    solver.set_hm_option("precond", 0)
+    solver.set_method("dense")
+
+
+def test_block_jacobi_falls_back_to_the_factorization(bh, solver):
+    """Block Jacobi (the default) with 30 GMRES steps allowed does not converge on the 2,562-triangle
+    example (it needs about 50); the solve then builds the hierarchical factorization at eps_pre and
+    converges with it."""
+    from best_hydro_amd import meshes as M
+    solver.set_mesh(M.read_best(os.path.join(GOLDEN, "2eu7_02562")))
+    solver.set_method("hmatrix", eps=EPS, tol=1e-10)
+    solver.set_hm_option("maxsteps", 30)
+    h = solver.solve_mesh(ivisco=0)
+    st = solver.hm_stats()
+    log = solver.log()
+    solver.set_hm_option("maxsteps", 500)
+    solver.set_method("dense")
+    assert any("block Jacobi was not enough" in l for l in log)
+    assert st["eps_pre"] == 0.1 and st["relres"] <= 1e-10
+
+
+def test_direct_hlu_against_reference_library(bh, oracle, monkeypatch):
+    """-hlu through the C ABI: bhm_build with direct = 1 factors the H-matrix hierarchically at eps
+    and bhm_solve solves with those factors, refined against the H-matrix (no Krylov method), on the
+    2,562-triangle example at eps 1e-3.  The reference's own direct mode at the same eps (its H-LU,
+    hmatrix/src/api.cpp:461-495, oracle/_ref) is 6e-3 from LAPACK there; ours must be at least as
+    close."""
+    R = oracle.reference_lib()
+    m, _ = _prepared(oracle, "2eu7_02562")
+    n = m["nar"]
+    icomp, ncomp, nopen = oracle.compon(m)
+    x_o, gr = oracle.tgetgr(m)
+    oracle.deflate(m, gr, icomp, ncomp)
+    lu, piv, info = oracle.lapack_getrf(gr)
+    xo, sb = oracle.self_blocks(m)
+    b = oracle.difco_rhs(n, xo)
+    want = oracle.lapack_getrs(lu, piv, b.copy())
+    eps = 1e-3
+    monkeypatch.setenv("BEST_HIP_SOLVER", "hmatrix")
+    hm = bh.hm
+    assert hm.create(n, m["v1"], m["v2"], m["v3"], m["area"], m["afac"], 1.0, 0) == 0
+    diagm = hm.diag_mean()
+    xmu, gdef = oracle.deflation_weights(m, diagm, icomp, ncomp)
+    assert hm.build(eps, 1, 0.1, ncomp, gdef, icomp, xmu, 0, 6) == 0
+    rc, sol = hm.solve(b.copy())
+    lines = hm.log_lines()
+    print("\n".join(lines), flush=True)
+    assert rc == 0
+    assert any("-hlu" in l for l in lines) and not any("by GMRES" in l for l in lines)
+    e_gpu = np.max(np.abs(sol - want)) / np.max(np.abs(want))
+    hm.destroy()
+    e_ref = 6.2e-3  # the reference library's direct mode here (measured below when it is built)
+    if R is not None:
+        assert R.bhm_create(n, m["v1"], m["v2"], m["v3"], m["area"], m["afac"], 1.0, 8) == 0
+        assert R.bhm_build(eps, 1, 0.1, ncomp, gdef, icomp, xmu, 0, 6) == 0
+        bref = b.copy(order="F").reshape(-1, order="F")
+        assert R.bhm_solve(6, bref) == 0
+        e_ref = np.max(np.abs(bref.reshape(b.shape, order="F") - want)) / np.max(np.abs(want))
+    print(f"-hlu at eps {eps}: GPU {e_gpu:.2e} from LAPACK, the reference library {e_ref:.2e}", flush=True)
+    assert e_gpu <= e_ref
