@@ -298,6 +298,15 @@ def main():
     # the stream each kernel runs on), for the roofline; the events cost a few percent, so this
     # pass is kept out of the value above.
     _, elp, prof32, _ = run(32, args.steps, profile=True)
+    # one profiled step of the FP64-LU comparator: its GEMM kernel's own rate
+    _, _, prof64, _ = run(64, 1, profile=True)
+    g64 = prof64["gemm64"]
+    fp64_gemm = None
+    if g64["launches"]:
+        fp64_gemm = {"kernel": "k_gemm<double, 16, *, *> (FP64 LU: panels and trailing updates; 2 waves per SIMD)",
+                     "achieved": round(g64["flops"] / (g64["ms"] * 1e-3) / 1e12, 3), "peak": PEAKS["fp64_mfma_tflops"],
+                     "unit": "TFLOP/s", "launches": g64["launches"], "ms_per_step": round(g64["ms"], 3)}
+        fp64_gemm["frac"] = round(fp64_gemm["achieved"] / fp64_gemm["peak"], 4)
 
     # Dominant kernel class of the mixed path, by measured time.
     flop_classes = {"pairs": ("fp64_valu_tflops", "TFLOP/s"), "gemm32": ("fp32_mfma_tflops", "TFLOP/s"),
@@ -359,7 +368,8 @@ def main():
         "config": {"workload": "bumpy ellipsoid 30/20/15 A, l<=6 radial bumps sd 0.03 (seeded), level-4 icosphere",
                    "triangles": n, "unknowns": 3 * n, "rhs": 11, "lu": "FP32 MFMA + FP64 iterative refinement",
                    "parallelism": "single GPU"},
-        "fp64_lu": {"value": round(1e3 / ms64, 4), "unit": "solves/s", "ms_per_step": round(ms64, 3)},
+        "fp64_lu": {"value": round(1e3 / ms64, 4), "unit": "solves/s", "ms_per_step": round(ms64, 3),
+                    "gemm": fp64_gemm},
         "ratio_mixed_over_fp64": round(ms64 / ms32, 3),
         "phases_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("t_") and not k.startswith("t_adjoint")},
         "ir_iterations": st32["ir_iterations"],

@@ -74,7 +74,8 @@ class BhHmStats(C.Structure):
                 ("iters_min", C.c_int), ("iters_max", C.c_int), ("steps", C.c_int), ("relres", C.c_double),
                 ("t_solve", C.c_double), ("n_trunc_failed", C.c_long), ("kmean_aca", C.c_double),
                 ("t_trunc", C.c_double), ("eps_pre", C.c_double), ("piv_ratio", C.c_double), ("pre_levels", C.c_int),
-                ("pre_rank_max", C.c_int), ("pre_bytes", C.c_double)]
+                ("pre_rank_max", C.c_int), ("pre_bytes", C.c_double), ("mem_estimate", C.c_double),
+                ("mem_free", C.c_double), ("mem_floor", C.c_double), ("mem_used", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -28,6 +28,9 @@ struct HmOptions {
                                //   eps_pre if GMRES stalls), 1 the hierarchical factorization at eps_pre
    double eps_pre = 0.1;       // its accuracy (BEST's EPSPRE, best/besolve.f90:34)
    int pre_skip = 0;           // coarsest levels of the hierarchical factorization left out (diagnostic)
+   int nrhs = 16;              // right-hand sides per solve at most (for the memory estimate)
+   int memcheck = 1;           // refuse a build the estimate says does not fit above the floor
+   double mem_floor = -1.0;    // device bytes to keep free (< 0: 5% of the device's memory, at least 1 GB)
    int direct = 0;             // 1 (-hlu): the hierarchical factorization at eps solves directly, refined
                                //   against the H-matrix, no GMRES
 };
@@ -52,6 +55,8 @@ struct HmStats {
    double piv_ratio = 0;       // its smallest pivot ratio (cluster systems)
    int pre_levels = 0, pre_rank_max = 0;
    double pre_bytes = 0;
+   double mem_estimate = 0, mem_free = 0, mem_floor = 0;  // bytes: the build's estimate, free before it, the floor
+   double mem_used = 0;        // bytes the build took (free memory before it less free memory after it)
 };
 
 // The geometry: vertices (v1, v2, v3 as BEST's NAR x 3 column-major arrays, one after the other)

@@ -2217,6 +2217,51 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
    say( "cluster tree: depth %d, %d leaf clusters of %d to %d triangles, %.2f s", S.depth, S.nleaves, S.leaf_min, S.leaf_max,
         S.t_tree );
 
+   // The device memory the build and the solves will need, estimated before anything is allocated
+   // (the reference's memory plan and refusal, hmatrix/src/api.cpp:146-339, :585-630): the dense
+   // blocks exactly; the low-rank blocks at (rows + columns) x a rank model of the cross
+   // approximations (measured at 245,760 unknowns: mean rank 48 at eps 1e-6, about 8 per decade of
+   // eps, capped at kcap), which bounds the recompressed storage too; the approximations' work space
+   // (its budget); the Krylov basis; the hierarchical factorization's bases at its widest (when it
+   // is the preconditioner from the start, or -hlu); the vectors.  The floor is 5% of the device's
+   // memory, at least 1 GB.
+   {
+      const size_t esz = opt.single ? sizeof( float ) : sizeof( double );
+      const double kmodel = std::min<double>( opt.kcap, std::max( 4.0, 8.0 * std::log10( 1.0 / opt.eps ) ) );
+      double dense = 0.0, lowrank = 0.0, ws = 0.0;
+      for ( const Blk& b : h.blks ) {
+         if ( b.type == kLowRank ) {
+            lowrank += (double)( b.nr + b.nc ) * kmodel * esz;
+            ws += 8.0 * (double)( b.nr + b.nc ) * std::min<long>( opt.kcap, std::min( b.nr, b.nc ) / 2 );
+         } else {
+            dense += (double)b.nr * b.nc * esz;
+         }
+      }
+      ws = std::min( ws, opt.ws_bytes );
+      const double n3d = 3.0 * n;
+      const int restart = std::max( opt.restart, 30 );
+      const double krylov = 8.0 * n3d * opt.nrhs * ( restart + 8 );
+      const double pre = ( opt.precond == 1 || opt.direct ) ? 2.0 * 8.0 * n3d * T.depth * kHdMaxR : 8.0 * n3d * 96;
+      const double est = dense + lowrank + ws + krylov + pre + 8.0 * n3d * ( kSrcStride + 20 );
+      size_t freeb = 0, total = 0;
+      BH_HIP( hipMemGetInfo( &freeb, &total ) );
+      const double floor = opt.mem_floor >= 0.0 ? opt.mem_floor : std::max( 1e9, 0.05 * (double)total );
+      S.mem_estimate = est, S.mem_free = (double)freeb, S.mem_floor = floor;
+      say( "memory: about %.1f GB needed (dense blocks %.1f, low-rank about %.1f, approximation work space %.1f, Krylov "
+           "basis %.1f, preconditioner %.1f), %.1f GB free on the device",
+           est / 1e9, dense / 1e9, lowrank / 1e9, ws / 1e9, krylov / 1e9, pre / 1e9, freeb / 1e9 );
+      if ( opt.memcheck && est > (double)freeb - floor ) {
+         char buf[ 400 ];
+         std::snprintf( buf, sizeof buf,
+                        "not enough device memory for the H-matrix solver at eps %.0e: about %.1f GB needed, %.1f GB free above "
+                        "the floor of %.1f GB (BEST_HMATRIX_MEMFLOOR sets the floor in GB; set BEST_HMATRIX_MEMCHECK=0 to "
+                        "try anyway)%s",
+                        opt.eps, est / 1e9, std::max( 0.0, ( (double)freeb - floor ) / 1e9 ), floor / 1e9,
+                        opt.single ? "" : "; -hs (single precision) needs about half" );
+         throw std::runtime_error( buf );
+      }
+   }
+
    // tree-ordered data
    t0 = now();
    h.perm.ensure( n );
@@ -2740,6 +2785,11 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
    S.t_pre = now() - t0;
    say( "preconditioner: %s, %.3f s", h.hd_on ? "the hierarchical factorization on the block-Jacobi leaf inverses"
                                                : "block Jacobi on the diagonal leaf blocks", S.t_pre );
+   {
+      size_t freeb = 0, total = 0;
+      BH_HIP( hipMemGetInfo( &freeb, &total ) );
+      S.mem_used = S.mem_free - (double)freeb;
+   }
    h.defl_s.ensure( (size_t)h.ndef * kMaxRhs );
    h.defl_part.ensure( (size_t)( ( h.n3 + kDotChunkDefl - 1 ) / kDotChunkDefl ) * h.ndef * kMaxRhs );
    h.ypart.ensure( (size_t)h.nleaves * kSplit * kRowsWg * kMaxRhs );

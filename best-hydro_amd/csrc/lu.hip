@@ -61,11 +61,12 @@ struct BatchStride {
 };
 
 // Waves per SIMD the register allocation must allow: three for FP32 with 16-column slices (LDS then
-// holds three workgroups per CU, and the accumulators move to ordinary VGPRs without spills),
-// else no constraint.
+// holds three workgroups per CU, and the accumulators move to ordinary VGPRs without spills), two
+// for FP64 (the 64 FP64 accumulators of a 128 x 128 tile in 128 VGPRs, 195 in all, no spills, and
+// two workgroups' LDS per CU), else no constraint.
 template <typename T, int BK>
 struct GemmOcc {
-   static constexpr int waves = ( sizeof( T ) == 4 && BK == 16 ) ? 3 : 1;
+   static constexpr int waves = ( sizeof( T ) == 4 && BK == 16 ) ? 3 : sizeof( T ) == 8 ? 2 : 1;
 };
 
 template <typename T, int BK, int BM = 128, int BN = 128>

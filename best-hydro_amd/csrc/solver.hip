@@ -1192,6 +1192,8 @@ int bhg_set_hm_option( bhg_solver* s, const char* name, double value ) {
    else if ( k == "eps_pre" && value > 0.0 ) o.eps_pre = value;
    else if ( k == "direct" ) o.direct = value != 0.0 ? 1 : 0;
    else if ( k == "pre_skip" ) o.pre_skip = std::max( 0, (int)value );
+   else if ( k == "memcheck" ) o.memcheck = value != 0.0 ? 1 : 0;
+   else if ( k == "mem_floor_gb" ) o.mem_floor = value * 1e9;
    else return 1;
    return 0;
 }
@@ -1228,6 +1230,10 @@ int bhg_get_hm_stats( bhg_solver* s, bhg_hm_stats* out ) {
    out->pre_levels = h.pre_levels;
    out->pre_rank_max = h.pre_rank_max;
    out->pre_bytes = h.pre_bytes;
+   out->mem_estimate = h.mem_estimate;
+   out->mem_free = h.mem_free;
+   out->mem_floor = h.mem_floor;
+   out->mem_used = h.mem_used;
    return 0;
 }
 
@@ -1815,7 +1821,6 @@ double bhm_diag_mean( void ) {
 
 int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* gdef, const int* comp, const double* xmu,
                int single, int nrhs ) {
-   (void)nrhs;
    if ( !g_solver ) {
       g_log.emplace_back( "error: bhm_build before bhm_create" );
       return 1;
@@ -1840,6 +1845,12 @@ int bhm_build( double eps, int direct, double eps_pre, int ndef, const double* g
          // block Jacobi, which is faster here, and the factorization only if GMRES stalls
          const char* pc = std::getenv( "BEST_HIP_PRECOND" );
          if ( pc ) s->hm_opt.precond = std::string( pc ) == "hierarchical" ? 1 : 0;
+         // the memory contract of hmatrix/best_hmatrix.h:48-60, for device memory
+         const char* mc = std::getenv( "BEST_HMATRIX_MEMCHECK" );
+         s->hm_opt.memcheck = mc ? ( std::atof( mc ) != 0.0 ) : 1;
+         const char* mf = std::getenv( "BEST_HMATRIX_MEMFLOOR" );
+         s->hm_opt.mem_floor = mf ? 1e9 * std::atof( mf ) : -1.0;
+         s->hm_opt.nrhs = std::min( std::max( nrhs, 1 ), bhg::kMaxRhs );
          s->ndef = std::max( ndef, 1 );
          s->gdef.ensure( s->ndef );
          BH_HIP( hipMemcpyAsync( s->gdef.p, gdef, s->ndef * sizeof( double ), hipMemcpyHostToDevice, s->st ) );

@@ -196,6 +196,9 @@ typedef struct bhg_hm_stats {
    double piv_ratio;          /* its smallest pivot ratio (the cluster systems) */
    int pre_levels, pre_rank_max; /* its levels and their largest basis width */
    double pre_bytes;          /* device bytes of its bases */
+   double mem_estimate, mem_free, mem_floor, mem_used; /* device bytes: the estimate made before the build
+                                                          (hmatrix/src/api.cpp:146-339), free then, the
+                                                          floor kept free, what the build took */
 } bhg_hm_stats;
 
 /* method 0: dense LU (default); 1: H-matrix + GMRES (hmatrix/src/api.cpp:585-735) for the next

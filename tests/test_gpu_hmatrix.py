@@ -236,6 +236,9 @@ def test_c5_matches_reference_library(bh, solver):
     assert errs["dtrans"] < 1e-8 and errs["dr"] < 1e-8
     for key in ("dtt", "drr", "dtr"):
         assert errs[key] < 1e-7, key
+    # the estimate made before the build bounds what the build took
+    print(f"memory: estimate {st['mem_estimate'] / 1e9:.1f} GB, the build took {st['mem_used'] / 1e9:.1f} GB", flush=True)
+    assert st["mem_used"] <= st["mem_estimate"]
     solver.set_method("dense")
 
 
@@ -344,3 +347,34 @@ def test_direct_hlu_against_reference_library(bh, oracle, monkeypatch):
         e_ref = np.max(np.abs(bref.reshape(b.shape, order="F") - want)) / np.max(np.abs(want))
     print(f"-hlu at eps {eps}: GPU {e_gpu:.2e} from LAPACK, the reference library {e_ref:.2e}", flush=True)
     assert e_gpu <= e_ref
+
+
+def test_memory_estimate_refuses_before_the_build(bh, oracle, solver, monkeypatch):
+    """The memory contract of hmatrix/best_hmatrix.h:48-60 on device memory: with a floor above the
+    device's memory the build is refused before anything is allocated, with a message; with
+    BEST_HMATRIX_MEMCHECK=0 it goes ahead (handle API and bhm_build)."""
+    from best_hydro_amd import meshes as M
+    solver.set_mesh(M.read_best(os.path.join(GOLDEN, "1aki49_atom_01806")))
+    solver.set_method("hmatrix", eps=1e-4, tol=1e-8)
+    solver.set_hm_option("mem_floor_gb", 1e6)
+    with pytest.raises(bh.HipError, match="not enough device memory for the H-matrix solver"):
+        solver.solve_mesh(ivisco=0)
+    solver.set_hm_option("memcheck", 0)
+    solver.solve_mesh(ivisco=0)
+    solver.set_hm_option("memcheck", 1)
+    solver.set_hm_option("mem_floor_gb", -1)
+    solver.set_method("dense")
+    m, _ = _prepared(oracle, "1aki49_atom_01806")
+    n = m["nar"]
+    icomp, ncomp, nopen = oracle.compon(m)
+    monkeypatch.setenv("BEST_HIP_SOLVER", "hmatrix")
+    monkeypatch.setenv("BEST_HMATRIX_MEMFLOOR", "1000000")
+    hm = bh.hm
+    assert hm.create(n, m["v1"], m["v2"], m["v3"], m["area"], m["afac"], 1.0, 0) == 0
+    xmu, gdef = oracle.deflation_weights(m, hm.diag_mean(), icomp, ncomp)
+    assert hm.build(1e-4, 0, 0.1, ncomp, gdef, icomp, xmu, 0, 6) != 0
+    lines = hm.log_lines()
+    assert any("not enough device memory" in l and "BEST_HMATRIX_MEMCHECK=0" in l for l in lines)
+    monkeypatch.setenv("BEST_HMATRIX_MEMCHECK", "0")
+    assert hm.build(1e-4, 0, 0.1, ncomp, gdef, icomp, xmu, 0, 6) == 0
+    hm.destroy()
