@@ -55,6 +55,12 @@ class BhResult(C.Structure):
         return out
 
 
+class BhBeParams(C.Structure):
+    _fields_ = [("ver", C.c_char_p), ("iarea", C.c_int), ("truearea", C.c_double), ("truevol", C.c_double),
+                ("mw", C.c_int), ("wm", C.c_double), ("nar", C.c_int), ("ivisco", C.c_int), ("icov", C.c_int),
+                ("cg", C.c_double)]
+
+
 class BhStats(C.Structure):
     _fields_ = [("t_prep", C.c_double), ("t_assemble", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_reduce", C.c_double), ("t_total", C.c_double), ("ir_iterations", C.c_int),
@@ -178,6 +184,10 @@ def lib():
     L.bhg_hm_blocks.restype = i
     L.bhg_gradient.argtypes = [vp, i, d, i, C.POINTER(BhResult), _dp]
     L.bhg_gradient.restype = i
+    L.bhg_format_be.argtypes = [C.POINTER(BhResult), C.POINTER(BhBeParams), C.c_char_p, lg, C.c_char_p, i]
+    L.bhg_format_be.restype = lg
+    L.bhg_write_be.argtypes = [C.c_char_p, C.POINTER(BhResult), C.POINTER(BhBeParams), C.c_char_p, i]
+    L.bhg_write_be.restype = i
     _LIB = L
     return L
 
@@ -464,6 +474,50 @@ def extrapolate_1n(n, y):
     if lib().bhg_extrapolate_1n(len(n), n, y, C.byref(a), C.byref(b)) != 0:
         raise HipError("extrapolate_1n: fewer than two meshes or equal N")
     return a.value, b.value
+
+
+BEST_VERSION = "4.0 SFSU/ARAGON"  # best/best.f:180
+
+
+def _be_args(result, nar, truearea, truevol, iarea, mw, wm, ivisco, icov, cg, ver):
+    r = BhResult()
+    for name, _ in BhResult._fields_:
+        if name not in result:
+            continue
+        v = result[name]
+        if isinstance(getattr(r, name), C.Array):
+            a = np.asarray(v, dtype=np.float64)
+            a = a.reshape(-1, order="F") if a.ndim == 2 else a.reshape(-1)
+            getattr(r, name)[:] = [float(t) for t in a]
+        else:
+            setattr(r, name, type(getattr(r, name))(v))
+    p = BhBeParams(ver.encode(), int(iarea), float(truearea), float(truevol), int(mw), float(wm), int(nar),
+                   int(ivisco), int(icov), float(cg))
+    return r, p
+
+
+def format_be(result, nar, truearea=0.0, truevol=0.0, iarea=1, mw=0, wm=0.0, ivisco=1, icov=0, cg=1.0,
+              ver=BEST_VERSION) -> bytes:
+    """The .be file BEST's OUTPUT (best/output.f) writes for a solve's result (a ``solve_mesh``
+    dict), byte for byte (bhg_format_be).  Raises HipError when the tensors or the viscosity
+    values are not finite, as BEST stops (best/best.f:483-496)."""
+    r, p = _be_args(result, nar, truearea, truevol, iarea, mw, wm, ivisco, icov, cg, ver)
+    err = C.create_string_buffer(256)
+    n = lib().bhg_format_be(C.byref(r), C.byref(p), None, 0, err, 256)
+    if n < 0:
+        raise HipError(err.value.decode())
+    buf = C.create_string_buffer(n + 1)
+    lib().bhg_format_be(C.byref(r), C.byref(p), buf, n + 1, err, 256)
+    return buf.raw[:n]
+
+
+def write_be(path, result, nar, truearea=0.0, truevol=0.0, iarea=1, mw=0, wm=0.0, ivisco=1, icov=0, cg=1.0,
+             ver=BEST_VERSION) -> None:
+    """format_be's text written to path (bhg_write_be)."""
+    r, p = _be_args(result, nar, truearea, truevol, iarea, mw, wm, ivisco, icov, cg, ver)
+    err = C.create_string_buffer(256)
+    if lib().bhg_write_be(os.fsencode(path), C.byref(r), C.byref(p), err, 256) != 0:
+        raise HipError(err.value.decode())
 
 
 def hm_plan(v1, v2, v3, x, leaf=32, eta=2.0):

@@ -2779,7 +2779,7 @@ HMatrixPtr hm_build( hipStream_t st, hipStream_t st_b, hipStream_t st_c, const H
       } else if ( opt.direct ) {
          throw std::runtime_error( "H-LU: the factorization broke down; without -hlu BEST solves by GMRES" );
       } else {
-         say( "preconditioner: block Jacobi on the diagonal leaf blocks alone" );
+         say( "%s", "preconditioner: block Jacobi on the diagonal leaf blocks alone" );
       }
    }
    S.t_pre = now() - t0;

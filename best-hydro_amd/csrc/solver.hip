@@ -139,7 +139,7 @@ struct bhg_solver {
    bool a32_fresh = false;    // a32 holds the FP32 rounding of the assembled a64 (the pair kernel stored both)
    bool transposed = false;   // the factors (and the FP64 matrix) are transposed in place: solves are with GR^T
    // vectors
-   bhg::DevBuf<double> b, x, r, y64, x64il, p64, part, norms, sums;
+   bhg::DevBuf<double> b, x, r, y64, x64il, p64, part, norms, sums, trdev;
    bhg::DevBuf<float> d32, y32, x32, p32;
    // adjoint (bhg_gradient)
    bhg::DevBuf<double> adj_F, adj_y, adj_lam, adj_ghat, adj_gsrc_part, adj_gcol_part, adj_gsrc, adj_gcol;
@@ -153,6 +153,8 @@ struct bhg_solver {
    double fp32_growth = bhg::kFp32Growth;  // largest multiplier the exchange-free FP32 LU accepts
    int ir_steps_nopiv = 0;                 // refinement steps with those factors before the pivoted
                                            // refactorization (0: ir_maxit)
+   int host_algebra = 0;                   // 1: the reductions summed and BEST's algebra done on the
+                                           // host (a check of the device algebra)
    const bhg::SolveWork* work_override = nullptr;  // solve_device's work vectors when not the buffers above
 
    void say( const char* fmt, ... ) __attribute__( ( format( printf, 2, 3 ) ) ) {
@@ -372,6 +374,7 @@ void size_vectors( bhg_solver* s ) {
    s->part.ensure( residual_part_elems( s->npad ) );
    s->norms.ensure( 64 );
    s->sums.ensure( (size_t)kSumGroups * kSumSlots );
+   s->trdev.ensure( sizeof( Transport ) / sizeof( double ) );
 
    if ( grow ) {
       BH_HIP( hipMemsetAsync( s->b.p, 0, need * sizeof( double ), s->st ) );
@@ -519,33 +522,8 @@ void collect_sums( bhg_solver* s, double* out ) {
    }
 }
 
-// The DIFCO/VISCO reductions of the solutions in s->x and BEST's algebra on them (best/difco.f,
-// best/visco.f, best/output.f) into out.  Synchronizes.
-void compute_results( bhg_solver* s, int iarea, double cg, int ivisco, int icov, bhg_result* out ) {
-   const int n = s->n;
-   const long np = s->npad;
-   const double zero[ 3 ] = { 0.0, 0.0, 0.0 };
-   launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, ivisco ? 5 : 0, zero, icov ? 1 : 0, s->sums.p );
-   double sums[ kSumSlots ];
-   collect_sums( s, sums );
-   Transport tr {};
-   difco_algebra( sums, cg, tr );
-   if ( ivisco ) {
-      if ( icov ) {
-         visco_centre( sums, tr.dvisc );
-         launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, 5, tr.dvisc, 0, s->sums.p );
-         double sums2[ kSumSlots ];
-         collect_sums( s, sums2 );
-         tr.chiprime = visco_chiprime( sums2, cg );
-      } else {
-         tr.chiprime = visco_chiprime( sums, cg );
-      }
-   }
-   output_values( tr );
-   double scal[ 8 ];
-   BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, s->st ) );
-   BH_HIP( hipStreamSynchronize( s->st ) );
-   if ( !out ) return;
+// out from the transport values and the prep's scalars (afac, total area, centroid, diagm).
+void fill_result( const bhg_solver* s, int iarea, const Transport& tr, const double* scal, bhg_result* out ) {
    std::memcpy( out->dtt, tr.dtt, sizeof tr.dtt );
    std::memcpy( out->dtr, tr.dtr, sizeof tr.dtr );
    std::memcpy( out->drr, tr.drr, sizeof tr.drr );
@@ -568,6 +546,53 @@ void compute_results( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
    out->ncomp = s->ncomp;
    out->nopen = s->nopen;
    out->ndef = s->ndef;
+}
+
+// Copies the transport values the device computed (launch_transport into s->trdev) and the prep's
+// scalars to the host, into out.  Synchronizes.
+void read_results( bhg_solver* s, int iarea, bhg_result* out ) {
+   Transport tr {};
+   double scal[ 8 ];
+   BH_HIP( hipMemcpyAsync( &tr, s->trdev.p, sizeof tr, hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipStreamSynchronize( s->st ) );
+   if ( out ) fill_result( s, iarea, tr, scal, out );
+}
+
+// The DIFCO/VISCO reductions of the solutions in s->x and BEST's algebra on them (best/difco.f,
+// best/visco.f, best/output.f) into out, on the device; with host_algebra the partial sums are
+// added and the algebra done on the host instead.  Synchronizes.
+void compute_results( bhg_solver* s, int iarea, double cg, int ivisco, int icov, bhg_result* out ) {
+   const int n = s->n;
+   const long np = s->npad;
+   if ( !s->host_algebra ) {
+      launch_transport( s->st, n, s->xc.p, s->area.p, s->x.p, np, ivisco, icov, cg, s->sums.p,
+                        reinterpret_cast<Transport*>( s->trdev.p ) );
+      read_results( s, iarea, out );
+      return;
+   }
+   const double zero[ 3 ] = { 0.0, 0.0, 0.0 };
+   launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, ivisco ? 5 : 0, zero, icov ? 1 : 0, s->sums.p );
+   double sums[ kSumSlots ];
+   collect_sums( s, sums );
+   Transport tr {};
+   difco_algebra( sums, cg, tr );
+   if ( ivisco ) {
+      if ( icov ) {
+         visco_centre( sums, tr.dvisc );
+         launch_sums( s->st, n, s->xc.p, s->area.p, s->x.p, np, 5, tr.dvisc, 0, s->sums.p );
+         double sums2[ kSumSlots ];
+         collect_sums( s, sums2 );
+         tr.chiprime = visco_chiprime( sums2, cg );
+      } else {
+         tr.chiprime = visco_chiprime( sums, cg );
+      }
+   }
+   output_values( tr );
+   double scal[ 8 ];
+   BH_HIP( hipMemcpyAsync( scal, s->scal.p, sizeof scal, hipMemcpyDeviceToHost, s->st ) );
+   BH_HIP( hipStreamSynchronize( s->st ) );
+   if ( out ) fill_result( s, iarea, tr, scal, out );
 }
 
 // Tile choices of one mesh of a series (picked by the autotuner).
@@ -598,7 +623,7 @@ void prepare_buffers( bhg_solver* s ) {
 // with no host interaction: nothing here reads a device value, allocates or sets memory, so the
 // sequence can be captured in a HIP graph.  The pivot check and the refinement's convergence are
 // read afterwards (finish_enqueued).
-void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshTune& t ) {
+void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, int icov, const MeshTune& t ) {
    const int n = s->n;
    const long np = s->npad;
    g_gemm_bk_f32 = t.gemm_bk;
@@ -620,12 +645,16 @@ void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, const MeshT
    launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
    const int nrhs = ivisco ? 11 : 6;
    for ( int it = 0; it < t.ir_iters; ++it ) ir_step( s, nrhs, it );
+   // the reductions and the algebra on the solutions as they stand (used when no fallback runs)
+   launch_transport( s->st, n, s->xc.p, s->area.p, s->x.p, np, ivisco, icov, cg, s->sums.p,
+                     reinterpret_cast<Transport*>( s->trdev.p ) );
    g_gemm_bk_f32 = 0;
 }
 
 // After an enqueued solve has run: if partial pivoting would have exchanged rows, the mesh is
-// solved again on the pivoted path; if the refinement has not converged it continues.  Then the
-// reductions and BEST's algebra.
+// solved again on the pivoted path; if the refinement has not converged it continues, and the
+// reductions and BEST's algebra are done again on its solutions.  Otherwise the enqueued ones
+// stand.
 void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov, const MeshTune& t, bhg_result* out ) {
    const int nrhs = ivisco ? 11 : 6;
    s->factored = 32;
@@ -633,6 +662,7 @@ void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
    s->transposed = false;
    s->mesh_ready = true;
    s->stats.ir_above_tol = 0;
+   bool redo = true;
    if ( read_flag( s ) ) {
       s->say( "series: a multiplier exceeded %g; the mesh is solved again with row exchanges", s->fp32_growth );
       assemble( s, true, true );
@@ -643,14 +673,17 @@ void finish_enqueued( bhg_solver* s, int iarea, double cg, int ivisco, int icov,
       const IrCorrection c = ir_correction( s, nrhs );
       s->stats.ir_iterations = t.ir_iters;
       s->stats.ir_correction = c.worst;
-      if ( !ir_converged( c, t.ir_iters - 1, s->ir_tol ) && !ir_loop( s, nrhs, t.ir_iters ) ) {
+      if ( ir_converged( c, t.ir_iters - 1, s->ir_tol ) ) {
+         redo = s->host_algebra != 0;
+      } else if ( !ir_loop( s, nrhs, t.ir_iters ) ) {
          refactor_pivoted32( s );
          if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
       }
    }
    s->stats.pivoted = s->pivoted ? 1 : 0;
    s->stats.npad = s->npad;
-   compute_results( s, iarea, cg, ivisco, icov, out );
+   if ( redo ) compute_results( s, iarea, cg, ivisco, icov, out );
+   else read_results( s, iarea, out );
 }
 
 // One dense (or H-matrix) solve of the resident mesh: prep, self terms, assembly, factorization,
@@ -1159,6 +1192,8 @@ int bhg_set_option( bhg_solver* s, const char* name, double value ) {
       s->fp32_growth = value;
    } else if ( k == "ir_steps_before_pivoting" && value >= 0.0 ) {
       s->ir_steps_nopiv = (int)value;
+   } else if ( k == "host_algebra" && ( value == 0.0 || value == 1.0 ) ) {
+      s->host_algebra = (int)value;
    } else {
       s->last_error = "bhg_set_option: unknown option or value out of range: " + k;
       return 1;
@@ -1490,7 +1525,7 @@ struct bhg_series {
    hipGraph_t graph = nullptr;
    hipGraphExec_t exec = nullptr;
    int parallel = 1;            // meshes side by side in the schedule (1) or one after another (0)
-   int key_iarea = -1, key_ivisco = -1;
+   int key_iarea = -1, key_ivisco = -1, key_icov = -1;
    double key_cg = -1.0;
    std::string last_error;
    void drop_graph() {
@@ -1516,7 +1551,7 @@ namespace {
 
 // Captures the solves of every mesh (enqueue_solve) as one HIP graph: forked from the series
 // stream to each mesh's streams and joined back, side by side or chained one after another.
-void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int parallel ) {
+void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int icov, int parallel ) {
    BH_HIP( hipSetDevice( S->device ) );
    S->drop_graph();
    for ( auto* m : S->m ) prepare_buffers( m );
@@ -1529,7 +1564,7 @@ void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int parall
          bhg_solver* m = S->m[ k ];
          BH_HIP( hipEventRecord( S->ev[ 2 * k ], parallel ? S->so : prev ) );
          BH_HIP( hipStreamWaitEvent( m->st, S->ev[ 2 * k ], 0 ) );
-         enqueue_solve( m, iarea, cg, ivisco, S->tune[ k ] );
+         enqueue_solve( m, iarea, cg, ivisco, icov, S->tune[ k ] );
          BH_HIP( hipEventRecord( S->ev[ 2 * k + 1 ], m->st ) );
          prev = m->st;
       }
@@ -1549,6 +1584,7 @@ void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int parall
    S->key_iarea = iarea;
    S->key_cg = cg;
    S->key_ivisco = ivisco;
+   S->key_icov = icov;
 }
 
 // One launch of the captured schedule; the device time in ms.
@@ -1717,14 +1753,14 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
       for ( int kset : { 0, 1 } ) {
          S->tune = kset ? k32 : tuned;
          for ( int par : { 1, 0 } ) {
-            bhg::series_capture( S, iarea, cg, ivisco, par );
+            bhg::series_capture( S, iarea, cg, ivisco, 0, par );
             bhg::series_replay( S );
             const double ms = bhg::series_replay( S );
             if ( ms < best ) best = ms, choice = par, choice_k = kset;
          }
       }
       S->tune = choice_k ? k32 : tuned;
-      bhg::series_capture( S, iarea, cg, ivisco, choice );
+      bhg::series_capture( S, iarea, cg, ivisco, 0, choice );
    } );
 }
 
@@ -1732,8 +1768,8 @@ int bhg_series_solve( bhg_series* S, int iarea, double cg, int ivisco, int icov,
    return bhg::series_guarded( S, "bhg_series_solve", [&] {
       for ( auto* m : S->m )
          if ( !m->mesh_raw ) throw std::runtime_error( "a mesh of the series is not set" );
-      if ( !S->exec || S->key_iarea != iarea || S->key_cg != cg || S->key_ivisco != ivisco )
-         bhg::series_capture( S, iarea, cg, ivisco, S->parallel );
+      if ( !S->exec || S->key_iarea != iarea || S->key_cg != cg || S->key_ivisco != ivisco || S->key_icov != icov )
+         bhg::series_capture( S, iarea, cg, ivisco, icov, S->parallel );
       const double ms = bhg::series_replay( S );
       if ( graph_ms ) *graph_ms = ms;
       for ( size_t k = 0; k < S->m.size(); ++k )

@@ -1,4 +1,4 @@
-// Transport properties from the solutions of BEST's 11 flows (host-side declarations).
+// Transport properties from the solutions of BEST's 11 flows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,9 +22,14 @@ struct Transport {
 void launch_rhs( hipStream_t st, int n, const double* xc, double* b, long ldb, int with_strain );
 void launch_sums( hipStream_t st, int n, const double* xc, const double* area, const double* b, long ldb, int nstrain,
                   const double d[ 3 ], int want_bb, double* partial );
-void difco_algebra( const double* sums, double cg, Transport& T );
-void visco_centre( const double* sums, double d[ 3 ] );
-double visco_chiprime( const double* sums, double cg );
-void output_values( Transport& T );
+__host__ __device__ void difco_algebra( const double* sums, double cg, Transport& T );
+__host__ __device__ void visco_centre( const double* sums, double d[ 3 ] );
+__host__ __device__ double visco_chiprime( const double* sums, double cg );
+__host__ __device__ void output_values( Transport& T );
+void jacobi3( const double* a, double d[ 3 ] );  // host
+// The reductions and the whole algebra on the device, into d_out (device memory): partial holds
+// kSumGroups x kSumSlots doubles.
+void launch_transport( hipStream_t st, int n, const double* xc, const double* area, const double* b, long ldb, int ivisco,
+                       int icov, double cg, double* partial, Transport* d_out );
 
 }  // namespace bhg

@@ -1,5 +1,6 @@
-// The right-hand sides BEST solves for and the reductions of its solutions, on the GPU; the 3x3
-// algebra that turns the reductions into BEST's tensors, on the host (FP64 throughout).
+// The right-hand sides BEST solves for, the reductions of its solutions and the 3x3 algebra that
+// turns the reductions into BEST's tensors, all on the GPU (FP64 throughout; the algebra is also
+// callable on the host, where the adjoint differentiates it).
 //
 //   k_rhs           best/difco.f:28-44 (six rigid-body flows) and best/visco.f:216-243 (STRAIN,
 //                   five strain flows), as 11 columns of one right-hand-side block
@@ -9,7 +10,10 @@
 //   difco_algebra   best/difco.f:106-401 with MATIN3 (best/matin3.f)
 //   visco_chiprime  best/visco.f:118-211
 //   output_values   best/output.f:16-29, :98-167 with JACOBI (best/jacobi.f)
+//   k_transport     the partial sums added in workgroup order, then the algebra above, on one
+//                   thread: the whole of DIFCO/VISCO/OUTPUT's arithmetic without a host round trip
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -55,9 +59,11 @@ __global__ void k_rhs( int n, const double* __restrict__ xc, double* __restrict_
 //   [45, 54)  BB(1,1), BB(1,2), BB(1,3), BB(2,2), BB(2,3), BB(3,3), C1, C2, C3 (before scaling)
 __global__ void __launch_bounds__( 256 ) k_sums( int n, const double* __restrict__ xc, const double* __restrict__ area,
                                                 const double* __restrict__ b, long ldb, int nstrain, double d1, double d2,
-                                                double d3, int want_bb, double* __restrict__ partial ) {
+                                                double d3, const double* __restrict__ dptr, int want_bb,
+                                                double* __restrict__ partial ) {
    __shared__ double red[ 256 ];
    double acc[ kSumSlots ];
+   if ( dptr ) d1 = dptr[ 0 ], d2 = dptr[ 1 ], d3 = dptr[ 2 ];  // the centre computed on the device
    for ( int k = 0; k < kSumSlots; ++k ) acc[ k ] = 0.0;
    const int per = ( n + gridDim.x - 1 ) / gridDim.x;
    const int lo = blockIdx.x * per, hi = min( n, lo + per );
@@ -158,27 +164,36 @@ void launch_rhs( hipStream_t st, int n, const double* xc, double* b, long ldb, i
 void launch_sums( hipStream_t st, int n, const double* xc, const double* area, const double* b, long ldb, int nstrain,
                   const double d[ 3 ], int want_bb, double* partial ) {
    hipLaunchKernelGGL( k_sums, dim3( kSumGroups ), dim3( 256 ), 0, st, n, xc, area, b, ldb, nstrain, d[ 0 ], d[ 1 ], d[ 2 ],
-                       want_bb, partial );
+                       nullptr, want_bb, partial );
 }
 
-// ------------------------------------------------------------------ host 3x3 algebra
+// ------------------------------------------------------------------ 3x3 algebra (device and host)
 
 namespace {
 
+// No contraction of a * b + c into fused multiply-adds in the algebra, on the device as on the
+// host: the two give the same bits from the same sums.
+#pragma clang fp contract( off )
+
+#define BH_HD __host__ __device__
+
 // 3x3 matrices column-major: m[i + 3 j] is (i, j), 0-based.
-inline double& at( double* m, int i, int j ) { return m[ i + 3 * j ]; }
-inline double at( const double* m, int i, int j ) { return m[ i + 3 * j ]; }
+BH_HD inline double& at( double* m, int i, int j ) { return m[ i + 3 * j ]; }
+BH_HD inline double at( const double* m, int i, int j ) { return m[ i + 3 * j ]; }
+BH_HD inline void copy9( double* d, const double* s ) {
+   for ( int k = 0; k < 9; ++k ) d[ k ] = s[ k ];
+}
 
 // The inverse of a 3x3 matrix by LU with implicit (scaled) partial pivoting and back
 // substitution per unit column (best/matin3.f).  a is overwritten.  Returns nonzero when a row is
 // all zeros.
-int inverse3( double* a, double* y ) {
+BH_HD int inverse3( double* a, double* y ) {
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) at( y, i, j ) = i == j ? 1.0 : 0.0;
    double scale[ 3 ];
    for ( int i = 0; i < 3; ++i ) {
       double big = 0.0;
-      for ( int j = 0; j < 3; ++j ) big = std::fabs( at( a, i, j ) ) > big ? std::fabs( at( a, i, j ) ) : big;
+      for ( int j = 0; j < 3; ++j ) big = fabs( at( a, i, j ) ) > big ? fabs( at( a, i, j ) ) : big;
       if ( big == 0.0 ) return 1;
       scale[ i ] = 1.0 / big;
    }
@@ -195,11 +210,15 @@ int inverse3( double* a, double* y ) {
          double s = at( a, i, j );
          for ( int k = 0; k < j; ++k ) s = s - at( a, i, k ) * at( a, k, j );
          if ( j > 0 ) at( a, i, j ) = s;
-         const double r = scale[ i ] * std::fabs( s );
+         const double r = scale[ i ] * fabs( s );
          if ( r >= big ) big = r, imax = i;
       }
       if ( imax != j ) {
-         for ( int k = 0; k < 3; ++k ) std::swap( at( a, imax, k ), at( a, j, k ) );
+         for ( int k = 0; k < 3; ++k ) {
+            const double t = at( a, imax, k );
+            at( a, imax, k ) = at( a, j, k );
+            at( a, j, k ) = t;
+         }
          scale[ imax ] = scale[ j ];
       }
       perm[ j ] = imax;
@@ -234,33 +253,33 @@ int inverse3( double* a, double* y ) {
 
 // Cyclic Jacobi eigenvalues of a symmetric 3x3 matrix read from its upper triangle
 // (best/jacobi.f).  a is destroyed.
-void eigen3( double* a, double d[ 3 ] ) {
+BH_HD void eigen3( double* a, double d[ 3 ] ) {
    double b[ 3 ], z[ 3 ];
    for ( int p = 0; p < 3; ++p ) b[ p ] = d[ p ] = at( a, p, p ), z[ p ] = 0.0;
    for ( int sweep = 1; sweep <= 50; ++sweep ) {
       double off = 0.0;
       for ( int p = 0; p < 2; ++p )
-         for ( int q = p + 1; q < 3; ++q ) off = off + std::fabs( at( a, p, q ) );
+         for ( int q = p + 1; q < 3; ++q ) off = off + fabs( at( a, p, q ) );
       if ( off == 0.0 ) return;
       const double tresh = sweep < 4 ? 0.2 * off / 9.0 : 0.0;
       for ( int p = 0; p < 2; ++p )
          for ( int q = p + 1; q < 3; ++q ) {
             const double apq = at( a, p, q );
-            const double g = 100.0 * std::fabs( apq );
-            if ( sweep > 4 && std::fabs( d[ p ] ) + g == std::fabs( d[ p ] ) && std::fabs( d[ q ] ) + g == std::fabs( d[ q ] ) ) {
+            const double g = 100.0 * fabs( apq );
+            if ( sweep > 4 && fabs( d[ p ] ) + g == fabs( d[ p ] ) && fabs( d[ q ] ) + g == fabs( d[ q ] ) ) {
                at( a, p, q ) = 0.0;
                continue;
             }
-            if ( !( std::fabs( apq ) > tresh ) ) continue;
+            if ( !( fabs( apq ) > tresh ) ) continue;
             double h = d[ q ] - d[ p ], t;
-            if ( std::fabs( h ) + g == std::fabs( h ) ) {
+            if ( fabs( h ) + g == fabs( h ) ) {
                t = apq / h;
             } else {
                const double theta = 0.5 * h / apq;
-               t = 1.0 / ( std::fabs( theta ) + std::sqrt( 1.0 + theta * theta ) );
+               t = 1.0 / ( fabs( theta ) + sqrt( 1.0 + theta * theta ) );
                if ( theta < 0.0 ) t = -t;
             }
-            const double c = 1.0 / std::sqrt( 1 + t * t ), s = t * c, tau = s / ( 1.0 + c );
+            const double c = 1.0 / sqrt( 1 + t * t ), s = t * c, tau = s / ( 1.0 + c );
             h = t * apq;
             z[ p ] = z[ p ] - h, z[ q ] = z[ q ] + h;
             d[ p ] = d[ p ] - h, d[ q ] = d[ q ] + h;
@@ -280,7 +299,7 @@ void eigen3( double* a, double d[ 3 ] ) {
 
 // uu = the matrix of the centre vector r of best/difco.f:180-188 (the cross-product matrix,
 // with r(1) at (3,2)).
-void cross_matrix( const double r[ 3 ], double* uu ) {
+BH_HD void cross_matrix( const double r[ 3 ], double* uu ) {
    at( uu, 2, 0 ) = -r[ 1 ];
    at( uu, 2, 1 ) = r[ 0 ];
    at( uu, 0, 1 ) = -r[ 2 ];
@@ -292,7 +311,7 @@ void cross_matrix( const double r[ 3 ], double* uu ) {
 
 }  // namespace
 
-void difco_algebra( const double* s, double cg, Transport& T ) {
+BH_HD void difco_algebra( const double* s, double cg, Transport& T ) {
    double ktt[ 9 ], ktr[ 9 ], krt[ 9 ], krr[ 9 ], uu[ 9 ], aux[ 9 ], kinv[ 9 ], work[ 9 ];
    // Friction tensors: column i of Ktt/Ktr from translation i, of Krt/Krr from rotation i.
    for ( int i = 0; i < 3; ++i )
@@ -302,7 +321,7 @@ void difco_algebra( const double* s, double cg, Transport& T ) {
          at( krt, j, i ) = -s[ 6 * ( 3 + i ) + j ] * ( 1.0 / 1.0 );
          at( krr, j, i ) = -s[ 6 * ( 3 + i ) + 3 + j ] * ( 1.0 / 1.0 );
       }
-   std::memcpy( T.ktr, ktr, sizeof ktr );
+   copy9( T.ktr, ktr );
    // Centre of resistance (difco.f:141-174).
    at( uu, 0, 0 ) = at( ktt, 1, 1 ) + at( ktt, 2, 2 );
    at( uu, 1, 1 ) = at( ktt, 0, 0 ) + at( ktt, 2, 2 );
@@ -334,11 +353,11 @@ void difco_algebra( const double* s, double cg, Transport& T ) {
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j )
          at( krt, i, j ) = at( krt, i, j ) + at( ktt, i, 0 ) * at( uu, 0, j ) + at( ktt, i, 1 ) * at( uu, 1, j ) + at( ktt, i, 2 ) * at( uu, 2, j );
-   std::memcpy( T.ktt, ktt, sizeof ktt );
-   std::memcpy( T.krt, krt, sizeof krt );
-   std::memcpy( T.krr, krr, sizeof krr );
+   copy9( T.ktt, ktt );
+   copy9( T.krt, krt );
+   copy9( T.krr, krr );
    // Dtt, Dtr, Drr with the coupling (difco.f:244-330).
-   std::memcpy( work, ktt, sizeof work );
+   copy9( work, ktt );
    inverse3( work, kinv );
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
@@ -399,7 +418,7 @@ void difco_algebra( const double* s, double cg, Transport& T ) {
                              at( uu, i, 2 ) * at( drr, 2, j );
 }
 
-void visco_centre( const double* s, double d[ 3 ] ) {
+BH_HD void visco_centre( const double* s, double d[ 3 ] ) {
    double bb[ 9 ], bbi[ 9 ];
    at( bb, 0, 0 ) = s[ 45 ] / 15.;
    at( bb, 0, 1 ) = s[ 46 ] / 30.;
@@ -417,7 +436,7 @@ void visco_centre( const double* s, double d[ 3 ] ) {
    d[ 2 ] = at( bbi, 2, 0 ) * c1 + at( bbi, 2, 1 ) * c2 + at( bbi, 2, 2 ) * c3;
 }
 
-double visco_chiprime( const double* s, double cg ) {
+BH_HD double visco_chiprime( const double* s, double cg ) {
    const double pi = 3.1415926535897932385;
    const double c112 = 0.5 * s[ 36 ], c213 = 0.5 * s[ 37 ], c323 = 0.5 * s[ 38 ];
    double chip = 8 * pi * ( 0.2 * ( c112 + c213 + c323 ) + ( s[ 39 ] - 2 * s[ 40 ] + s[ 41 ] + s[ 42 ] + s[ 43 ] - 2 * s[ 44 ] ) / 15.0 );
@@ -425,17 +444,72 @@ double visco_chiprime( const double* s, double cg ) {
    return chip;
 }
 
-void output_values( Transport& T ) {
+BH_HD void output_values( Transport& T ) {
    const double temperature = 293.15, eta = 0.01002, pi = 3.141592653;
    const double tunit = ( 1.38066E-16 ) * temperature * 1.0E8 / ( 8.0 * eta * pi );
    const double runit = tunit * 1.0e16;
    double a[ 9 ];
-   std::memcpy( a, T.drr, sizeof a );
+   copy9( a, T.drr );
    eigen3( a, T.drr_eig );
    T.dr = runit * ( ( T.drr_eig[ 0 ] + T.drr_eig[ 1 ] + T.drr_eig[ 2 ] ) / 3. );
-   std::memcpy( a, T.dtt, sizeof a );
+   copy9( a, T.dtt );
    eigen3( a, T.dtt_eig );
    T.dtrans = tunit * ( ( T.dtt_eig[ 0 ] + T.dtt_eig[ 1 ] + T.dtt_eig[ 2 ] ) / 3. );
+}
+
+// JACOBI's eigenvalues of a symmetric 3x3 matrix (upper triangle of a, column-major; a is not
+// modified), in JACOBI's order.
+void jacobi3( const double* a, double d[ 3 ] ) {
+   double w[ 9 ];
+   copy9( w, a );
+   eigen3( w, d );
+}
+
+// The kSumSlots sums from the workgroups' partials, each added in workgroup order (the order
+// the host used), on one thread.
+__device__ void total_sums( const double* __restrict__ partial, double* s ) {
+   for ( int k = 0; k < kSumSlots; ++k ) {
+      double t = 0.0;
+      for ( int g = 0; g < kSumGroups; ++g ) t = t + partial[ (long)g * kSumSlots + k ];
+      s[ k ] = t;
+   }
+}
+
+// DIFCO, VISCO (at the centroid, or the centre of viscosity when icov: the chiprime then comes
+// from k_transport_chi after the sums at that centre) and OUTPUT on one thread.
+__global__ void k_transport( const double* __restrict__ partial, int ivisco, int icov, double cg, Transport* out ) {
+   if ( blockIdx.x || threadIdx.x ) return;
+   double s[ kSumSlots ];
+   total_sums( partial, s );
+   Transport T {};
+   difco_algebra( s, cg, T );
+   T.chiprime = 0.0;
+   if ( ivisco ) {
+      if ( icov ) visco_centre( s, T.dvisc );
+      else T.chiprime = visco_chiprime( s, cg );
+   }
+   output_values( T );
+   *out = T;
+}
+
+__global__ void k_transport_chi( const double* __restrict__ partial, double cg, Transport* out ) {
+   if ( blockIdx.x || threadIdx.x ) return;
+   double s[ kSumSlots ];
+   total_sums( partial, s );
+   out->chiprime = visco_chiprime( s, cg );
+}
+
+void launch_transport( hipStream_t st, int n, const double* xc, const double* area, const double* b, long ldb, int ivisco,
+                       int icov, double cg, double* partial, Transport* d_out ) {
+   hipLaunchKernelGGL( k_sums, dim3( kSumGroups ), dim3( 256 ), 0, st, n, xc, area, b, ldb, ivisco ? 5 : 0, 0.0, 0.0, 0.0,
+                       nullptr, icov ? 1 : 0, partial );
+   hipLaunchKernelGGL( k_transport, dim3( 1 ), dim3( 1 ), 0, st, partial, ivisco, icov, cg, d_out );
+   if ( ivisco && icov ) {
+      const double* d_centre = reinterpret_cast<const double*>( reinterpret_cast<const char*>( d_out ) + offsetof( Transport, dvisc ) );
+      hipLaunchKernelGGL( k_sums, dim3( kSumGroups ), dim3( 256 ), 0, st, n, xc, area, b, ldb, 5, 0.0, 0.0, 0.0, d_centre, 0,
+                          partial );
+      hipLaunchKernelGGL( k_transport_chi, dim3( 1 ), dim3( 1 ), 0, st, partial, cg, d_out );
+   }
 }
 
 }  // namespace bhg

@@ -8,8 +8,8 @@ module best_hip
    use, intrinsic :: iso_c_binding
    implicit none
    private
-   public :: bhg_result, bhg_stats, bhg_create, bhg_destroy, bhg_set_mesh, bhg_solve_mesh, &
-             bhg_get_stats, bhg_get_solution, bhg_besolv, bhg_error
+   public :: bhg_result, bhg_stats, bhg_be_params, bhg_create, bhg_destroy, bhg_set_mesh, bhg_solve_mesh, &
+             bhg_get_stats, bhg_get_solution, bhg_besolv, bhg_error, bhg_write_be
 
    ! struct bhg_result: the values of BEST's .be file and the tensors behind them
    type, bind(C) :: bhg_result
@@ -30,7 +30,19 @@ module best_hip
       real(c_double) :: ir_correction
       integer(c_int) :: pivoted, npad
       real(c_double) :: t_adjoint_solve, t_adjoint_pairs, t_adjoint_tri
+      integer(c_int) :: ir_above_tol
    end type bhg_stats
+
+   ! struct bhg_be_params: what OUTPUT (best/output.f:1-3) takes besides the result
+   type, bind(C) :: bhg_be_params
+      type(c_ptr) :: ver
+      integer(c_int) :: iarea
+      real(c_double) :: truearea, truevol
+      integer(c_int) :: mw
+      real(c_double) :: wm
+      integer(c_int) :: nar, ivisco, icov
+      real(c_double) :: cg
+   end type bhg_be_params
 
    interface
       type(c_ptr) function bhg_create(device) bind(C, name='bhg_create')
@@ -78,6 +90,14 @@ module best_hip
          real(c_double), intent(inout) :: b(*)
          integer(c_long), value :: ldb
       end function bhg_besolv
+      integer(c_int) function bhg_write_be(path, res, par, err, errlen) bind(C, name='bhg_write_be')
+         import :: c_int, c_char, bhg_result, bhg_be_params
+         character(kind=c_char), intent(in) :: path(*)
+         type(bhg_result), intent(in) :: res
+         type(bhg_be_params), intent(in) :: par
+         character(kind=c_char), intent(out) :: err(*)
+         integer(c_int), value :: errlen
+      end function bhg_write_be
       integer(c_size_t) function c_strlen(p) bind(C, name='strlen')
          import :: c_ptr, c_size_t
          type(c_ptr), value :: p

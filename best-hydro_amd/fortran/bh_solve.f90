@@ -1,18 +1,25 @@
 ! bh_solve -- one BEST mesh solved on the GPU from Fortran, through the best_hip module.
 !
-! Usage: bh_solve MESHFILE [PRECISION [IAREA [IVISCO [ICOV]]]]
+! Usage: bh_solve MESHFILE [PRECISION [IAREA [IVISCO [ICOV [WM]]]]]
 !   MESHFILE   BEST's input: NAR TRUEAREA TRUEVOL, then 9 vertex coordinates per triangle
 !              (best/best.f:189-219)
 !   PRECISION  32 (FP32 LU on the matrix cores + FP64 refinement, default) or 64 (FP64 LU)
 !   IAREA      1 area correction (BEST's default), 0 as -na;  IVISCO 1 as -v;  ICOV 1 as -vc
-! Prints D_t (cm^2/s), D_r (1/s), CHI, CHI*V and the centre vectors in full precision.
+!   WM         molecular weight as -mw (g/mol; 0 or absent: none)
+! Prints D_t (cm^2/s), D_r (1/s), CHI, CHI*V and the centre vectors in full precision, and writes
+! BEST's .be file beside the mesh, named as BEST names it (best/best.f:224-249: the mesh file name,
+! then v, c, m, n for -v, -vc, -mw, -na, then .be).
 program bh_solve
    use, intrinsic :: iso_c_binding
    use best_hip
    implicit none
    character(len=512) :: fname, arg
    integer :: nar, n, prec, iarea, ivisco, icov, rc
-   real(c_double) :: truearea, truevol
+   real(c_double) :: truearea, truevol, wm
+   character(len=600) :: befile
+   character(kind=c_char, len=256), target :: err
+   character(kind=c_char, len=17), target :: ver
+   type(bhg_be_params) :: par
    real(c_double), allocatable :: v1(:,:), v2(:,:), v3(:,:)
    type(c_ptr) :: s
    type(bhg_result) :: r
@@ -42,6 +49,11 @@ program bh_solve
    if (command_argument_count() >= 5) then
       call get_command_argument(5, arg)
       read(arg, *) icov
+   end if
+   wm = 0.0_c_double
+   if (command_argument_count() >= 6) then
+      call get_command_argument(6, arg)
+      read(arg, *) wm
    end if
 
    open(10, file=trim(fname), status='old', action='read')
@@ -76,5 +88,29 @@ program bh_solve
    print '(a,3es24.16)', 'URC (A)         ', r%urc
    print '(a,3es24.16)', 'DVISC (A)       ', r%dvisc
    print '(a,f10.3,a,i0)', 'total ms ', st%t_total, '  refinement steps ', st%ir_iterations
+
+   befile = trim(fname)
+   if (ivisco == 1) befile = trim(befile)//'v'
+   if (icov == 1) befile = trim(befile)//'c'
+   if (wm > 0.0_c_double) befile = trim(befile)//'m'
+   if (iarea == 0) befile = trim(befile)//'n'
+   befile = trim(befile)//'.be'
+   ver = '4.0 SFSU/ARAGON'//c_null_char
+   par%ver = c_loc(ver)
+   par%iarea = int(iarea, c_int)
+   par%truearea = truearea
+   par%truevol = truevol
+   par%mw = merge(1_c_int, 0_c_int, wm > 0.0_c_double)
+   par%wm = wm
+   par%nar = int(nar, c_int)
+   par%ivisco = int(ivisco, c_int)
+   par%icov = int(icov, c_int)
+   par%cg = 1.0_c_double
+   rc = bhg_write_be(trim(befile)//c_null_char, r, par, err, int(len(err), c_int))
+   if (rc /= 0) then
+      print '(2a)', 'bh_solve: ', err(1:index(err, c_null_char) - 1)
+      stop 1
+   end if
+   print '(2a)', 'wrote ', trim(befile)
    call bhg_destroy(s)
 end program bh_solve

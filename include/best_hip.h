@@ -284,6 +284,27 @@ int bhg_series_get_tuning( bhg_series* s, int k, int* out );
 int bhg_series_solve( bhg_series* s, int iarea, double cg, int ivisco, int icov, bhg_result* out, double* graph_ms );
 int bhg_series_get_stats( bhg_series* s, int k, bhg_stats* out );
 
+/* ---- the .be file ---------------------------------------------------------------------------
+ * BEST's OUTPUT (best/output.f:1-176, called at best/best.f:498) writes one mesh's results to the
+ * .be file; these two write the same records, byte for byte, from a bhg_result.  What OUTPUT takes
+ * besides the result (best/output.f:1-3): */
+typedef struct bhg_be_params {
+   const char* ver;           /* VER, padded or cut to 16 characters (BEST: "4.0 SFSU/ARAGON", best/best.f:180) */
+   int iarea;                 /* IAREA: 1 prints True Area, Areafactor, True Volume and CHI */
+   double truearea, truevol;  /* TRUEAREA, TRUEVOL (A^2, A^3) */
+   int mw;                    /* MW: 1 prints the molecular weight and ETA */
+   double wm;                 /* WM (g/mol) */
+   int nar;                   /* NAR: triangles */
+   int ivisco, icov;          /* IVISCO, ICOV: the viscosity block, at the centre of viscosity */
+   double cg;                 /* CG: 0 prints the pre-averaging line */
+} bhg_be_params;
+/* The .be text into buf (len bytes with the terminating NUL; NULL or 0 to size it): returns the
+ * length of the whole text, or -1 with the message in err when the tensors or the viscosity values
+ * are not finite (best/best.f:483-496: BEST then stops and the .be file stays empty). */
+long bhg_format_be( const bhg_result* r, const bhg_be_params* p, char* buf, long len, char* err, int errlen );
+/* The same text written to path (replaced); 0 or 1 with the message in err. */
+int bhg_write_be( const char* path, const bhg_result* r, const bhg_be_params* p, char* err, int errlen );
+
 /* Least-squares line y = intercept + slope / N through nmesh points: the 1/N extrapolation. */
 int bhg_extrapolate_1n( int nmesh, const int* n, const double* y, double* intercept, double* slope );
 

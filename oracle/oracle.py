@@ -12,7 +12,13 @@ LAPACK, with the C textbook LU (``bo_getrf``) as an independent cross-check for 
 ``reference_lib()`` loads ``oracle/_ref/libbhm_ref.so``: the reference's own hmatrix library built
 from its sources by ``make -C oracle ref``.  Its ``bhm_entries``/``bhm_incenters``/``bhm_diag_mean``
 pin the oracle's matrix entries bit for bit, and its ``bhm_build``/``bhm_solve`` are the H-matrix
-CPU reference.
+CPU reference.  ``output_ref()`` loads ``oracle/_ref/libbest_output_ref.so``: the reference's
+``OUTPUT`` and ``JACOBI`` (``best/output.f``, ``best/jacobi.f``) compiled by the same recipe with
+amdflang; they pin ``format_be`` byte for byte and ``jacobi`` bit for bit.
+
+``format_be`` (pure Python, below) restates ``OUTPUT``'s records and the Fortran edit descriptors
+they use (Ew.d, Fw.d, Iw and list-directed character output as the Fortran runtime here writes
+them).
 """
 from __future__ import annotations
 
@@ -97,6 +103,53 @@ def reference_lib():
         R.bhm_log_line.restype = i
         _REF = R
     return _REF
+
+
+_OUT = None
+
+
+def output_ref():
+    """The reference's OUTPUT and JACOBI compiled from best/output.f and best/jacobi.f, or None."""
+    global _OUT
+    if _OUT is None:
+        path = os.path.join(HERE, "_ref", "libbest_output_ref.so")
+        if not os.path.exists(path):
+            return None
+        _OUT = C.CDLL(path)
+    return _OUT
+
+
+def ref_output_be(path, ver, iarea, truearea, afac, truevol, mw, wm, nar, centroid, urc, udr, dtt, dtr, drr,
+                  ivisco, icov, dvisc, chiprime, cg, totarea) -> bytes:
+    """The .be file the reference's OUTPUT writes (best/output.f:1-176) for these values; path is a
+    scratch file name (OUTPUT opens it with status 'old')."""
+    R = output_ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libbest_output_ref.so is not built")
+    d = lambda x: C.byref(C.c_double(float(x)))
+    i = lambda x: C.byref(C.c_int(int(x)))
+    keep = [np.asfortranarray(np.asarray(a, dtype=np.float64)).copy(order="F") for a in (urc, udr, dtt, dtr, drr)]
+    fn = path.encode().ljust(80)
+    if len(fn) > 80:
+        raise ValueError("path longer than 80 characters")
+    open(path, "wb").close()
+    ptr = [a.ctypes.data_as(C.c_void_p) for a in keep]
+    R.output_(fn, ver.encode().ljust(16)[:16], i(iarea), d(truearea), d(afac), d(truevol), i(mw), d(wm), i(nar),
+              d(centroid[0]), d(centroid[1]), d(centroid[2]), *ptr, i(ivisco), i(icov), d(dvisc[0]), d(dvisc[1]),
+              d(dvisc[2]), d(chiprime), d(cg), d(totarea), C.c_long(80), C.c_long(16))
+    with open(path, "rb") as f:
+        return f.read()
+
+
+def ref_jacobi(a):
+    """The reference's JACOBI (best/jacobi.f) on a 3x3 symmetric matrix: the eigenvalues."""
+    R = output_ref()
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64)).copy(order="F")
+    dd, v, nrot = np.zeros(3), np.zeros((3, 3), order="F"), C.c_int(0)
+    n = C.c_int(3)
+    R.jacobi_(a.ctypes.data_as(C.c_void_p), C.byref(n), C.byref(n), dd.ctypes.data_as(C.c_void_p),
+              v.ctypes.data_as(C.c_void_p), C.byref(nrot))
+    return dd
 
 
 def ref_log(R) -> list[str]:
@@ -344,3 +397,131 @@ def best_run(v1, v2, v3, truearea=0.0, truevol=0.0, iarea=1, cg=1.0, ivisco=1, i
     if keep:
         res["gr_head"] = gr_raw_head
     return res
+
+
+# ------------------------------------------------------------------ the .be file (best/output.f)
+
+def _fortran_e(x, w, d):
+    """Ew.d output editing: [-]0.x1..xd followed by E+zz (|exponent| <= 99) or +zzz (<= 999),
+    right-justified in w; the leading zero dropped when w has no room for it, asterisks when the
+    field does not fit; Inf and NaN right-justified."""
+    x = float(x)
+    if x != x:
+        t = "NaN"
+        return t.rjust(w) if len(t) <= w else "*" * w
+    if x in (float("inf"), float("-inf")):
+        t = ("-" if x < 0 else "") + "Inf"
+        return t.rjust(w) if len(t) <= w else "*" * w
+    neg = x < 0 or (x == 0.0 and str(x).startswith("-"))
+    if x == 0.0:
+        digits, e = "0" * d, 0
+    else:
+        m, ee = ("%.*e" % (d - 1, abs(x))).split("e")
+        digits, e = m.replace(".", ""), int(ee) + 1
+    if abs(e) <= 99:
+        ex = "E" + ("-" if e < 0 else "+") + "%02d" % abs(e)
+    elif abs(e) <= 999:
+        ex = ("-" if e < 0 else "+") + "%03d" % abs(e)
+    else:
+        return "*" * w
+    t = ("-" if neg else "") + "0." + digits + ex
+    if len(t) > w:
+        t = ("-" if neg else "") + "." + digits + ex
+    return t.rjust(w) if len(t) <= w else "*" * w
+
+
+def _fortran_f(x, w, d):
+    """Fw.d output editing: [-]i.ffff rounded to d decimals, right-justified in w; the zero before
+    the point dropped when w has no room for it, asterisks when the field does not fit."""
+    x = float(x)
+    t = "%.*f" % (d, abs(x))
+    neg = x < 0 or (x == 0.0 and str(x).startswith("-"))
+    t = ("-" if neg else "") + t
+    if len(t) > w and t.lstrip("-").startswith("0."):
+        t = ("-" if neg else "") + t.lstrip("-")[1:]
+    return t.rjust(w) if len(t) <= w else "*" * w
+
+
+def _fortran_i(n, w):
+    t = str(int(n))
+    return t.rjust(w) if len(t) <= w else "*" * w
+
+
+def format_be(ver, iarea, truearea, afac, truevol, mw, wm, nar, centroid, urc, udr, dtt, dtr, drr, ivisco, icov,
+              dvisc, chiprime, cg, totarea) -> bytes:
+    """The records OUTPUT writes to the .be file (best/output.f:36-168), in order, with the
+    constants and the arithmetic of best/output.f:16-20 (tunit, runit), :99-106 and :121-128
+    (eigenvalues by JACOBI, 1/3 trace, anisotropy, unit conversions), :151-164 (CHI, ETA).
+    Tensors are 3x3 arrays (row j printed as dtt(j,1..3)); ver is padded or cut to 16 characters."""
+    E = _fortran_e
+    ld = lambda *items: " " + "".join(items)   # list-directed: a blank, then the characters
+    fmt340 = lambda a, b, c: "".join("   " + E(v, 12, 5) for v in (a, b, c))
+    temperature, eta, pi = 293.15, 0.01002, 3.141592653
+    tunit = (1.38066E-16) * temperature * 1.0E8 / (8.0 * eta * pi)
+    runit = tunit * 1.0e16
+    dtt, dtr, drr = (np.asarray(a, dtype=np.float64).reshape(3, 3) for a in (dtt, dtr, drr))
+    L = [ld("BEST Ver. ", (ver + " " * 16)[:16]), "", " BE - MACROMOLECULE TRANSPORT TENSORS", ""]
+    L.append("    " + " Area (A^2)            " + E(totarea, 12, 5))
+    if iarea == 1:
+        L.append("    " + " True Area (A^2)       " + E(truearea, 12, 5))
+        L.append("    " + " Areafactor                 " + _fortran_f(afac, 7, 4))
+        L.append("    " + " True Volume (A^3)     " + E(truevol, 12, 5))
+    if mw == 1:
+        L.append("    " + " Molecular Weight (g/mol)  " + _fortran_f(wm, 8, 1))
+    L.append("    " + " Temperature (K)           " + _fortran_f(temperature, 8, 3))
+    L.append("    " + " Solvent Viscosity (Poise) " + _fortran_f(eta, 8, 5))
+    L.append("    " + " Number of Surface Patches    " + _fortran_i(nar, 5))
+    L.append(ld(" "))
+    if cg == 0:
+        L.append(ld(" PRE-AVERAGING USED FOR THIS CALCULATION"))
+    L.append(ld(" "))
+    L.append(ld(" Centroid Coordinates (A)"))
+    L.append(fmt340(*centroid))
+    L.append(ld(" Vector between Center of Resistance/Centroid "))
+    L.append(fmt340(*urc))
+    L.append(ld(" Vector between Centers of Diffusion/Resistance"))
+    L.append(fmt340(*udr))
+    L += [ld(" "), ld(" Diffusion Tensors in Center of Mobility"), ld(" "), ld(" Dtt Tensor (1/A)")]
+    L += [fmt340(*dtt[j]) for j in range(3)]
+    L += [ld(" "), ld(" Symmetric Dtr Tensor (1/A^2)")]
+    L += [fmt340(*dtr[j]) for j in range(3)]
+    L += [ld(" "), ld("  Drr TENSOR (1/A^3)")]
+    L += [fmt340(*drr[j]) for j in range(3)]
+
+    def eig_block(t):
+        dd = jacobi(t)
+        trace = (dd[0] + dd[1] + dd[2]) / 3.
+        dl = np.sqrt(abs(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] - dd[0] * dd[1] - dd[0] * dd[2] - dd[1] * dd[2]))
+        return dd, trace, dl
+
+    dd, trace, dl = eig_block(drr)
+    L += [ld(" "), ld("EIGENVALUES OF Drr TENSOR (1/A^3)"), fmt340(*dd), "  1/3 Trace = " + E(trace, 12, 5),
+          "  Anisotropy=" + E(dl, 12, 5), ld(" "), ld("EIGENVALUES OF Drr TENSOR (1/s)"),
+          fmt340(runit * dd[0], runit * dd[1], runit * dd[2]),
+          "  1/3 Trace DRR(EXACT) =" + E(runit * trace, 15, 5) + " 1/s", ""]
+    dd, trace, dl = eig_block(dtt)
+    L += [ld("EIGENVALUES OF Dtt TENSOR (1/A)"), fmt340(*dd), "  1/3 Trace = " + E(trace, 12, 5),
+          "  Anisotropy=" + E(dl, 12, 5), ld(" "), ld("EIGENVALUES OF Dtt TENSOR (cm^2/s)"),
+          fmt340(tunit * dd[0], tunit * dd[1], tunit * dd[2]),
+          "  1/3 Trace DTT(EXACT) =" + E(tunit * trace, 15, 5) + " cm^2/s", ""]
+    if ivisco:
+        what = "INTRINSIC VISCOSITY" if mw == 1 else "VISCOSITY FACTOR"
+        if icov == 1:
+            L.append(ld(what + " AT THE CENTER OF VISCOSITY"))
+            L.append(ld(" Vector between Center of Viscosity/Centroid"))
+            L.append("   " + "  Dx = " + E(dvisc[0], 12, 5) + "  Dy = " + E(dvisc[1], 12, 5) + "  Dz = " +
+                     E(dvisc[2], 12, 5))
+        else:
+            L.append(ld(what + " AT THE CENTROID"))
+        if iarea == 1:
+            with np.errstate(all="ignore"):
+                chi = float(np.float64(chiprime) / np.float64(truevol))
+            L += ["", "     CHI               " + E(chi, 16, 6)]
+            L += ["     CHI*VOLUME (A^3)  " + E(chiprime, 16, 6), ""]
+        else:
+            L += ["", "     CHI*VOLUME (A^3)  " + E(chiprime, 16, 6), ""]
+        if mw == 1:
+            with np.errstate(all="ignore"):
+                eta_v = float(np.float64(chiprime) * 0.602214 / np.float64(wm))
+            L += ["     ETA (cm^3/g)      " + E(eta_v, 16, 6), ""]
+    return ("\n".join(L) + "\n").encode()

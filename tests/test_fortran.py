@@ -1,5 +1,6 @@
 """The Fortran host over iso_c_binding (best-hydro_amd/fortran): the best_hip module binds the
-handle API of include/best_hip.h, and bh_solve solves one BEST mesh file with it."""
+handle API of include/best_hip.h, and bh_solve solves one BEST mesh file with it and writes BEST's
+.be file (bhg_write_be)."""
 import os
 import subprocess
 
@@ -42,3 +43,36 @@ def test_fortran_solve_matches_oracle(oracle, prec):
     assert rel(got["DR"], ref["dr"]) < 1e-10
     assert rel(got["CHI*V"], ref["chiprime"]) < 1e-10
     assert rel(got["CHI"], ref["chi"]) < 1e-10
+
+
+@needs_exe
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["2eu7_v_mw30000", "2eu7_vc_mw30000", "2eu7_na_v_mw30000"])
+def test_fortran_writes_the_be_file(bh, oracle, tmp_path, case):
+    """bh_solve -v -mw 30000 (and -vc, -na) on the reference's example mesh: the .be file is named
+    as BEST names it, its printed values carry the reference's expected_best_*.txt digits, and its
+    bytes equal the library's writer on the same solve driven from Python."""
+    import shutil
+    from conftest import golden_expected
+    from test_befile import check_against_fixture
+    f = golden_expected(case)["flags"]
+    mesh = tmp_path / f["file"]
+    shutil.copy(os.path.join(GOLDEN, f["file"]), mesh)
+    args = [EXE, str(mesh), "32", str(f["iarea"]), "1", str(f["icov"]), str(f["wm"])]
+    print(" ".join(args), flush=True)
+    p = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    suffix = "v" + ("c" if f["icov"] else "") + "m" + ("n" if f["iarea"] == 0 else "")
+    be = tmp_path / (f["file"] + suffix + ".be")
+    assert be.exists(), p.stdout
+    text = be.read_bytes()
+    check_against_fixture(oracle, text.decode(), case)
+    from best_hydro_amd import meshes as M
+    t = M.read_best(str(mesh))
+    s = bh.Solver(0)
+    s.set_mesh(t)
+    r = s.solve_mesh(iarea=f["iarea"], ivisco=1, icov=f["icov"], precision=32)
+    s.close()
+    want = bh.format_be(r, nar=t.nar, truearea=t.truearea, truevol=t.truevol, iarea=f["iarea"], mw=1, wm=f["wm"],
+                        ivisco=1, icov=f["icov"], cg=1.0)
+    assert text == want

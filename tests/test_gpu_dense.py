@@ -228,3 +228,28 @@ def test_row_exchanges_on_a_delaunay_mesh(bh, oracle, solver):
     for r in (r64, r32):
         for key in ("dtrans", "dr", "chiprime"):
             assert rel(r[key], ref[key]) < TOL, key
+
+
+@pytest.mark.parametrize("case", ["2eu7_v_mw30000", "2eu7_vc_mw30000", "2eu7_na_v_mw30000", "two_v_mw5000"])
+def test_be_file_of_the_gpu_solve(bh, oracle, solver, case):
+    """The .be file of the GPU's solve (FP32 LU + FP64 refinement, the device algebra) for each
+    case of the reference's expected_best_*.txt: the printed values carry the fixture's digits,
+    and every byte equals the .be the oracle's FP64 run gives."""
+    from conftest import golden_expected
+    from test_befile import check_against_fixture
+    from best_hydro_amd import meshes as M
+    f = golden_expected(case)["flags"]
+    t = M.read_best(os.path.join(GOLDEN, f["file"]))
+    print(f"{case}: GPU solve and oracle run", flush=True)
+    solver.set_mesh(t)
+    r = solver.solve_mesh(iarea=f["iarea"], ivisco=1, icov=f["icov"], precision=32)
+    text = bh.format_be(r, nar=t.nar, truearea=t.truearea, truevol=t.truevol, iarea=f["iarea"], mw=1, wm=f["wm"],
+                        ivisco=1, icov=f["icov"], cg=1.0)
+    check_against_fixture(oracle, text.decode(), case)
+    ref = oracle.best_run(t.v1, t.v2, t.v3, t.truearea, t.truevol, iarea=f["iarea"], cg=1.0, ivisco=1,
+                          icov=f["icov"], wm=f["wm"])
+    want = oracle.format_be("4.0 SFSU/ARAGON", f["iarea"], t.truearea, ref["afac"], t.truevol, 1, f["wm"], ref["nar"],
+                            ref["centroid"], ref["urc"], ref["udr"], ref["dtt"], ref["dtr"], ref["drr"], 1, f["icov"],
+                            ref["dvisc"], ref["chiprime"], 1.0, ref["totarea"])
+    diff = [(a, b) for a, b in zip(text.split(b"\n"), want.split(b"\n")) if a != b]
+    assert not diff, diff[:3]

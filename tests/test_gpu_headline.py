@@ -2,8 +2,10 @@
 (both LU precisions) and all eight meshes of the 1/N series in the captured graph the benchmark
 times, each against the oracle's dense FP64 path (LAPACK); the FP32 path's fallbacks (row
 exchanges after a large multiplier, after a refinement that does not converge, and the series'
-re-solve), each forced by an option and checked against the oracle; and the transport algebra
-against what the mathematics fixes (tests/transport_checks.py).
+re-solve), each forced by an option and checked against the oracle; the transport algebra
+against what the mathematics fixes (tests/transport_checks.py); the device algebra (k_transport)
+against the same algebra on the host, bit for bit, and in the captured graph with the centre of
+viscosity against the oracle.
 
 Bars: D_t, D_r, CHI*V and the JACOBI eigenvalues within 1e-10 relative; the coupling tensor Dtr
 within 1e-10 of its largest entry (of 1e-3 sqrt(|Dtt| |Drr|) where it vanishes, as on a sphere); the centre of resistance within 1e-10 of the body's size; the
@@ -247,3 +249,52 @@ def test_transport_algebra_on_gpu(bh, oracle):
     assert mob["dtr"] < 1e-12 and mob["drr"] < 1e-12 and mob["dtt"] < mob["dtt_bound"]
     e_cr, b_cr, e_cd, b_cd = TC.centre_errors(r)
     assert e_cr < b_cr and e_cd < b_cd
+
+
+_KEYS = ("dtt", "dtr", "drr", "urc", "udr", "ktt", "krt", "krr", "drr_eig", "dtt_eig", "dr", "dtrans",
+         "chiprime", "chi", "dvisc")
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_device_algebra_matches_host(bh, prec):
+    """The reductions summed and BEST's algebra run on the device (k_transport), and the same on
+    the host (host_algebra 1) from the same solve: the same bits, with and without the move to
+    the centre of viscosity (the second reduction reads the centre from device memory)."""
+    import os
+    from conftest import GOLDEN
+    from best_hydro_amd import meshes as M
+    t = M.read_best(os.path.join(GOLDEN, "1aki49_atom_01806"))
+    s = bh.Solver(0)
+    s.set_mesh(t)
+    for icov in (0, 1):
+        print(f"precision {prec}, icov {icov}", flush=True)
+        rd = s.solve_mesh(ivisco=1, icov=icov, precision=prec)
+        s.set_option("host_algebra", 1)
+        rh = s.solve_mesh(ivisco=1, icov=icov, precision=prec)
+        s.set_option("host_algebra", 0)
+        for k in _KEYS:
+            assert np.array_equal(np.asarray(rd[k]), np.asarray(rh[k])), (icov, k)
+        assert (np.any(np.asarray(rd["dvisc"]) != 0.0)) == bool(icov)
+    s.close()
+
+
+def test_series_graph_centre_of_viscosity(bh, oracle):
+    """icov 1 in the captured graph (the centre computed and the second reduction run on the
+    device, no host step between): against the oracle's run with the centre of viscosity."""
+    import os
+    from conftest import GOLDEN
+    from best_hydro_amd import meshes as M
+    t = M.read_best(os.path.join(GOLDEN, "1aki49_atom_01806"))
+    sphere = M.sphere_mesh(3, 15.0)
+    ser = bh.Series([t, sphere])
+    for k in range(2):
+        ser.set_tuning(k, strip=64, gemm_bk=0, ir_iters=4)
+    ser.solve(ivisco=1, icov=1)
+    res, _ = ser.solve(ivisco=1, icov=1)
+    ser.close()
+    for r, m in zip(res, (t, sphere)):
+        ref = oracle.best_run(m.v1, m.v2, m.v3, m.truearea, m.truevol, iarea=1, cg=1.0, ivisco=1, icov=1)
+        _check_oracle(r, ref, m)
+        d = float(np.max(np.abs(np.asarray(r["dvisc"]) - np.asarray(ref["dvisc"]))))
+        print(f"{m.nar} triangles: centre of viscosity within {d:.1e} A of the oracle's", flush=True)
+        assert d < 1e-10 * _size(m)
