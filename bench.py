@@ -227,6 +227,19 @@ def counter_pass(s, M, steps):
     print("counter pass: three 1 GiB copies done", flush=True)
 
 
+def valu_pass(s, M):
+    """The command the assembly's VALU counters come from (rocprofv3 --pmc SQ_...): one mixed solve
+    of the workload (k_pairs: 5,120 x 5,120 pairs) and one gradient (a solve, then k_adj_pairs),
+    after a warm-up of each."""
+    for k in range(2):
+        r = s.solve_mesh(iarea=1, cg=1.0, ivisco=1, icov=0, precision=32)
+        print(f"valu pass: solve {k}: D_t {r['dtrans']:.10e}", flush=True)
+    for k in range(2):
+        r, _ = s.gradient(precision=32)
+        print(f"valu pass: gradient {k}: D_t {r['dtrans']:.10e}", flush=True)
+    s.close()
+
+
 def traffic_from_profile(kernel_class):
     """Calibrated HBM bytes per launch of the kernel class, from the committed counter summary
     (profiles/*_traffic.md, a JSON block written from two rocprofv3 --pmc runs), or None."""
@@ -255,6 +268,9 @@ def main():
     ap.add_argument("--counter-pass", action="store_true",
                     help="for rocprofv3 --pmc runs: --steps mixed solves of the workload, then three 1 GiB device "
                          "copies (the calibration of FETCH_SIZE / WRITE_SIZE); prints no JSON line")
+    ap.add_argument("--valu-pass", action="store_true",
+                    help="for rocprofv3 --pmc runs: two mixed solves and two gradients of the workload (the VALU "
+                         "counters of the assembly and of the adjoint's pair term); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve does not shard across "
@@ -273,6 +289,9 @@ def main():
     s.set_mesh(tri)  # inputs resident in HBM before any timing
     if args.counter_pass:
         counter_pass(s, M, args.steps)
+        return
+    if args.valu_pass:
+        valu_pass(s, M)
         return
 
     def run(prec, steps, profile=False):

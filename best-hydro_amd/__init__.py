@@ -119,6 +119,16 @@ def lib():
     L.bhg_compon.restype = i
     L.bhg_set_mesh.argtypes = [vp, i, _dp, _dp, _dp, d, d, vp]
     L.bhg_set_mesh.restype = i
+    L.bhg_set_mesh_file.argtypes = [vp, C.c_char_p, i]
+    L.bhg_set_mesh_file.restype = i
+    L.bhg_get_components.argtypes = [vp, C.POINTER(i), vp, C.POINTER(i), C.POINTER(i)]
+    L.bhg_get_components.restype = i
+    L.bhg_series_set_mesh_file.argtypes = [vp, i, C.c_char_p, i]
+    L.bhg_series_set_mesh_file.restype = i
+    L.bhg_read_best.argtypes = [C.c_char_p, i, i, C.POINTER(i), C.POINTER(d), C.POINTER(d), vp, vp, vp, C.c_char_p, i]
+    L.bhg_read_best.restype = i
+    L.bhg_write_best.argtypes = [C.c_char_p, i, d, d, _dp, _dp, _dp, C.c_char_p, i]
+    L.bhg_write_best.restype = i
     L.bhg_solve_mesh.argtypes = [vp, i, d, i, i, i, C.POINTER(BhResult)]
     L.bhg_solve_mesh.restype = i
     L.bhg_get_solution.argtypes = [vp, i, _dp, lg]
@@ -245,6 +255,22 @@ class Solver:
             ic = self._icomp.ctypes.data_as(C.c_void_p)
         self._check(self.L.bhg_set_mesh(self.h, self.n, _col(tri.v1), _col(tri.v2), _col(tri.v3), float(tri.truearea),
                                         float(tri.truevol), ic), "set_mesh")
+
+    def set_mesh_file(self, path, iarea=1):
+        """BEST's mesh file read by the library (bhg_set_mesh_file) and made resident, COMPON on
+        the device."""
+        self._check(self.L.bhg_set_mesh_file(self.h, os.fsencode(path), int(iarea)), "set_mesh_file")
+        nar = C.c_int()
+        self._check(self.L.bhg_get_components(self.h, C.byref(nar), None, None, None), "get_components")
+        self.n = nar.value
+
+    def components(self):
+        """(ICOMP, NCOMP, NOPEN) of the resident mesh, from COMPON (bhg_get_components)."""
+        ic = np.zeros(self.n, dtype=np.int32)
+        nc, no = C.c_int(), C.c_int()
+        self._check(self.L.bhg_get_components(self.h, None, ic.ctypes.data_as(C.c_void_p), C.byref(nc), C.byref(no)),
+                    "get_components")
+        return ic, nc.value, no.value
 
     def solve_mesh(self, iarea=1, cg=1.0, ivisco=1, icov=0, precision=32, wm=None):
         """best/best.f:261-498 on the resident mesh; returns the .be values and tensors."""
@@ -427,6 +453,10 @@ class Series:
         if rc != 0:
             raise HipError(f"{what}: {self.L.bhg_series_last_error(self.h).decode(errors='replace')}")
 
+    def set_mesh_file(self, k, path, iarea=1):
+        """Mesh k from BEST's mesh file (bhg_series_set_mesh_file)."""
+        self._check(self.L.bhg_series_set_mesh_file(self.h, int(k), os.fsencode(path), int(iarea)), "set_mesh_file")
+
     def set_option(self, name, value):
         """bhg_set_option for every mesh (bhg_series_set_option)."""
         self._check(self.L.bhg_series_set_option(self.h, name.encode(), float(value)), f"set_option {name}")
@@ -474,6 +504,33 @@ def extrapolate_1n(n, y):
     if lib().bhg_extrapolate_1n(len(n), n, y, C.byref(a), C.byref(b)) != 0:
         raise HipError("extrapolate_1n: fewer than two meshes or equal N")
     return a.value, b.value
+
+
+def read_best_file(path, iarea=1):
+    """The mesh file as the library reads it (bhg_read_best: best/best.f:189-220) ->
+    meshes.Triangulation."""
+    from . import meshes
+    L = lib()
+    nar, ta, tv = C.c_int(), C.c_double(), C.c_double()
+    err = C.create_string_buffer(512)
+    if L.bhg_read_best(os.fsencode(path), int(iarea), 0, C.byref(nar), C.byref(ta), C.byref(tv), None, None, None,
+                       err, 512) != 0:
+        raise HipError(err.value.decode(errors="replace"))
+    n = nar.value
+    v = [np.zeros(3 * n) for _ in range(3)]
+    if L.bhg_read_best(os.fsencode(path), int(iarea), n, C.byref(nar), C.byref(ta), C.byref(tv),
+                       *[a.ctypes.data_as(C.c_void_p) for a in v], err, 512) != 0:
+        raise HipError(err.value.decode(errors="replace"))
+    f = lambda a: a.reshape(3, n).T.copy()
+    return meshes.Triangulation(f(v[0]), f(v[1]), f(v[2]), ta.value, tv.value, os.path.basename(path))
+
+
+def write_best_file(path, tri):
+    """The mesh as rcoal writes it (bhg_write_best: list-directed header, then 9e17.8 lines)."""
+    err = C.create_string_buffer(512)
+    if lib().bhg_write_best(os.fsencode(path), len(tri.v1), float(tri.truearea), float(tri.truevol), _col(tri.v1),
+                            _col(tri.v2), _col(tri.v3), err, 512) != 0:
+        raise HipError(err.value.decode(errors="replace"))
 
 
 BEST_VERSION = "4.0 SFSU/ARAGON"  # best/best.f:180

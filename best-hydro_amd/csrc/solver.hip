@@ -21,6 +21,8 @@
 #include "../../include/best_hip.h"
 #include "adjoint.h"
 #include "bh_common.h"
+#include "compon.h"
+#include "files.h"
 #include "hmatrix.h"
 #include "transport.h"
 
@@ -121,7 +123,9 @@ struct bhg_solver {
    bool mesh_raw = false;     // raw (unshifted) vertices are resident (bhg_set_mesh)
    bool mesh_ready = false;   // prepared vertices, areas, incenters and self blocks are resident
    bhg::DevBuf<double> raw_v, v, area, xc, selfb, xmu, src, gdef, scal;
-   bhg::DevBuf<int> comp;
+   bhg::DevBuf<int> comp;     // the deflation's surface of each triangle (all 1 when a surface is open)
+   bhg::DevBuf<int> icomp;    // COMPON's ICOMP
+   bhg::ComponWork compw;
    // matrix and factors
    bhg::DevBuf<double> a64, inv64, xl64, xu64, tmp64;
    bhg::DevBuf<float> a32, inv32, xl32, xu32, tmp32;
@@ -1109,27 +1113,58 @@ int bhg_set_mesh( bhg_solver* s, int n, const double* v1, const double* v2, cons
       BH_HIP( hipMemcpyAsync( s->raw_v.p, v1, bytes, hipMemcpyHostToDevice, s->st ) );
       BH_HIP( hipMemcpyAsync( s->raw_v.p + 3 * (size_t)n, v2, bytes, hipMemcpyHostToDevice, s->st ) );
       BH_HIP( hipMemcpyAsync( s->raw_v.p + 6 * (size_t)n, v3, bytes, hipMemcpyHostToDevice, s->st ) );
-      std::vector<int> comp( n );
+      s->icomp.ensure( n );
       int ncomp = 1, nopen = 0;
       if ( icomp ) {
-         std::memcpy( comp.data(), icomp, n * sizeof( int ) );
-         ncomp = *std::max_element( comp.begin(), comp.end() );
+         ncomp = *std::max_element( icomp, icomp + n );
+         BH_HIP( hipMemcpyAsync( s->icomp.p, icomp, n * sizeof( int ), hipMemcpyHostToDevice, s->st ) );
       } else {
-         bhg::compon( n, v1, v2, v3, comp.data(), &ncomp, &nopen );
+         // COMPON on the device, on the vertices after the centroid shift, where best/best.f:323
+         // calls it (the shift is the prep's, bit for bit; each solve repeats it from raw_v)
+         BH_HIP( hipMemcpyAsync( s->v.p, s->raw_v.p, 9 * (size_t)n * sizeof( double ), hipMemcpyDeviceToDevice, s->st ) );
+         bhg::launch_prep( s->st, n, bhg::V1( s ), bhg::V2( s ), bhg::V3( s ), 0, 0.0, s->area.p, s->scal.p );
+         bhg::device_compon( s->st, n, s->v.p, s->icomp.p, s->compw, &ncomp, &nopen );
       }
       // best/best.f:324-337: one deflation term per closed surface, one for all if any is open.
       s->ncomp = ncomp;
       s->nopen = nopen;
       s->ndef = 1;
-      if ( nopen > 0 ) std::fill( comp.begin(), comp.end(), 1 );
-      else if ( ncomp > 1 ) s->ndef = ncomp;
-      BH_HIP( hipMemcpyAsync( s->comp.p, comp.data(), n * sizeof( int ), hipMemcpyHostToDevice, s->st ) );
+      if ( nopen > 0 ) bhg::fill( s->st, s->comp.p, n, 1 );
+      else BH_HIP( hipMemcpyAsync( s->comp.p, s->icomp.p, n * sizeof( int ), hipMemcpyDeviceToDevice, s->st ) );
+      if ( nopen == 0 && ncomp > 1 ) s->ndef = ncomp;
       BH_HIP( hipStreamSynchronize( s->st ) );
       s->truearea = truearea;
       s->truevol = truevol;
       s->mesh_raw = true;
       s->mesh_ready = false;
       s->factored = 0;
+   } );
+}
+
+int bhg_set_mesh_file( bhg_solver* s, const char* path, int iarea ) {
+   if ( !s ) return 1;
+   bhg::BestMesh m;
+   try {
+      if ( !path ) throw std::runtime_error( "no path" );
+      m = bhg::read_best( path, iarea );
+   } catch ( const std::exception& e ) {
+      s->last_error = std::string( "bhg_set_mesh_file: " ) + e.what();
+      return 1;
+   }
+   return bhg_set_mesh( s, m.nar, m.v1.data(), m.v2.data(), m.v3.data(), m.truearea, m.truevol, nullptr );
+}
+
+int bhg_get_components( bhg_solver* s, int* nar, int* icomp, int* ncomp, int* nopen ) {
+   return guarded( s, "bhg_get_components", [&] {
+      if ( !s->mesh_raw ) throw std::runtime_error( "no mesh (bhg_set_mesh)" );
+      bhg::set_device( s );
+      if ( nar ) *nar = s->n;
+      if ( icomp ) {
+         BH_HIP( hipMemcpyAsync( icomp, s->icomp.p, s->n * sizeof( int ), hipMemcpyDeviceToHost, s->st ) );
+         BH_HIP( hipStreamSynchronize( s->st ) );
+      }
+      if ( ncomp ) *ncomp = s->ncomp;
+      if ( nopen ) *nopen = s->nopen;
    } );
 }
 
@@ -1641,6 +1676,14 @@ bhg_series* bhg_series_create( int device, int nmesh ) {
 void bhg_series_destroy( bhg_series* S ) { delete S; }
 
 const char* bhg_series_last_error( bhg_series* S ) { return S ? S->last_error.c_str() : "no series"; }
+
+int bhg_series_set_mesh_file( bhg_series* S, int k, const char* path, int iarea ) {
+   return bhg::series_guarded( S, "bhg_series_set_mesh_file", [&] {
+      if ( k < 0 || k >= (int)S->m.size() ) throw std::runtime_error( "mesh index out of range" );
+      S->drop_graph();
+      if ( bhg_set_mesh_file( S->m[ k ], path, iarea ) != 0 ) throw std::runtime_error( S->m[ k ]->last_error );
+   } );
+}
 
 int bhg_series_set_mesh( bhg_series* S, int k, int n, const double* v1, const double* v2, const double* v3, double truearea,
                          double truevol ) {

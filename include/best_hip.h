@@ -101,13 +101,20 @@ void bhg_destroy( bhg_solver* s );
 const char* bhg_last_error( bhg_solver* s );
 int bhg_log_line( bhg_solver* s, char* buf, int len );
 
-/* COMPON (best/compon.f90:19-115): icomp[n] in 1..ncomp, and the open-edge count. */
+/* COMPON (best/compon.f90:19-115) on the host: icomp[n] in 1..ncomp, and the open-edge count. */
 int bhg_compon( int n, const double* v1, const double* v2, const double* v3, int* icomp, int* ncomp, int* nopen );
 
 /* A mesh as BEST reads it (best/best.f:189-220, before the centroid shift), made resident.  icomp
- * may be NULL (COMPON is run here). */
+ * may be NULL: COMPON then runs on the device, on the copied vertices after the centroid shift
+ * (where best/best.f:323 calls it). */
 int bhg_set_mesh( bhg_solver* s, int n, const double* v1, const double* v2, const double* v3, double truearea,
                   double truevol, const int* icomp );
+/* The mesh file BEST reads (best/best.f:189-220: OPEN(10), READ(10,*) NAR[, TRUEAREA, TRUEVOL]
+ * with iarea 1, then one READ of 9 values per triangle), read on the host and made resident as
+ * bhg_set_mesh does, COMPON on the device. */
+int bhg_set_mesh_file( bhg_solver* s, const char* path, int iarea );
+/* The resident mesh's NAR, and COMPON's result for it: icomp[NAR] (may be NULL), NCOMP, NOPEN. */
+int bhg_get_components( bhg_solver* s, int* nar, int* icomp, int* ncomp, int* nopen );
 
 /* One mesh solve on the resident mesh: best/best.f:261-498 without file I/O (prep, TGETGR,
  * deflation, DGETRF, DIFCO, VISCO, the .be values).  iarea = 1 applies the area correction
@@ -262,6 +269,8 @@ const char* bhg_series_last_error( bhg_series* s );
 /* Mesh k as BEST reads it (as bhg_set_mesh). */
 int bhg_series_set_mesh( bhg_series* s, int k, int n, const double* v1, const double* v2, const double* v3,
                          double truearea, double truevol );
+/* Mesh k from BEST's mesh file (as bhg_set_mesh_file). */
+int bhg_series_set_mesh_file( bhg_series* s, int k, const char* path, int iarea );
 
 /* The autotuner: for each mesh, the pair kernel's strip of source triangles per workgroup (16, 32,
  * 64, 128) and the FP32 GEMM's K slice (16, 32, 64) that time fastest, the refinement steps an
@@ -304,6 +313,19 @@ typedef struct bhg_be_params {
 long bhg_format_be( const bhg_result* r, const bhg_be_params* p, char* buf, long len, char* err, int errlen );
 /* The same text written to path (replaced); 0 or 1 with the message in err. */
 int bhg_write_be( const char* path, const bhg_result* r, const bhg_be_params* p, char* err, int errlen );
+
+/* ---- the mesh file ---------------------------------------------------------------------------
+ * The triangulation as BEST reads it (best/best.f:189-220) and rcoal writes it
+ * (rcoal/scoalesce.f:891-899: the list-directed header NAR TRUEAREA TRUEVOL, then format(9e17.8)
+ * per triangle).  v1, v2, v3 are NAR x 3 column-major, as BEST's V1, V2, V3.
+ * bhg_read_best: with v1, v2 or v3 NULL only *nar (and the header's areas) are returned, to size
+ * the arrays; cap = the triangles the arrays hold.  iarea 1 reads TRUEAREA and TRUEVOL from the
+ * header, 0 reads NAR alone as BEST does with -na.  Nonzero with the message in err on a missing,
+ * short or malformed file (BEST stops there with the Fortran runtime's error). */
+int bhg_read_best( const char* path, int iarea, int cap, int* nar, double* truearea, double* truevol, double* v1, double* v2,
+                   double* v3, char* err, int errlen );
+int bhg_write_best( const char* path, int nar, double truearea, double truevol, const double* v1, const double* v2,
+                    const double* v3, char* err, int errlen );
 
 /* Least-squares line y = intercept + slope / N through nmesh points: the 1/N extrapolation. */
 int bhg_extrapolate_1n( int nmesh, const int* n, const double* y, double* intercept, double* slope );
