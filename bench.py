@@ -256,6 +256,22 @@ def traffic_from_profile(kernel_class):
     return None, None
 
 
+def valu_from_profile():
+    """The VALU counter records of k_pairs and k_adj_pairs from the committed summary
+    (profiles/*_valu.md, a JSON block written from rocprofv3 --pmc runs of --valu-pass), or {}."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_valu.md"))):
+        with open(path) as f:
+            m = re.search(r"```json\n(.*?)\n```", f.read(), re.S)
+        if m:
+            rec = json.loads(m.group(1))
+            for v in rec.values():
+                v["source"] = os.path.basename(path)
+            return rec
+    return {}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,8 +385,10 @@ def main():
         achieved = work / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
         return {"achieved": round(achieved, 2), "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
                 "ms": round(ms, 3)}
+    valu = valu_from_profile()
     phase_rooflines = {
-        "assembly_fp64_valu": phase(pair_flops, st32["t_assemble"], PEAKS["fp64_valu_tflops"], "TFLOP/s"),
+        "assembly_fp64_valu": dict(phase(pair_flops, st32["t_assemble"], PEAKS["fp64_valu_tflops"], "TFLOP/s"),
+                                   counters=valu.get("k_pairs")),
         "lu_fp32_mfma": phase(lu_flops, st32["t_factor"], PEAKS["fp32_mfma_tflops"], "TFLOP/s"),
         "lu_fp64_mfma": phase(lu_flops, st64["t_factor"], PEAKS["fp64_mfma_tflops"], "TFLOP/s"),
         "solves_and_refinement_hbm": phase(nsolve * f32_solve_bytes + (nsolve - 1) * resid_bytes, st32["t_solve"],
@@ -416,6 +434,7 @@ def main():
         out["series_c3"] = series_c3(bh, M, max(1, min(args.steps, 5)))
     if not args.no_adjoint:
         out["adjoint_c4"] = adjoint_c4(bh, M, max(1, min(args.steps, 5)))
+        out["adjoint_c4"]["k_adj_pairs_counters"] = valu.get("k_adj_pairs")
     if not args.no_hmatrix:
         out["hmatrix_c5"] = hmatrix_c5(bh, M, max(1, min(args.steps, 3)))
     print(json.dumps(out), flush=True)
