@@ -13,9 +13,9 @@
 //   k_transpose    in-place transpose of square blocks: the factors, the super-block inverses
 //                  and the FP64 matrix of the refinement, for the adjoint solve
 //   k_adj_pairs    sum_k lambda_k,i^T (dG_ij/dtheta) F_k,j over every pair, theta the source
-//                  triangle's 9 vertex coordinates (forward mode, one direction per pass through
-//                  the pair integral of best/triangleint.f), its corrected area (G is linear in it)
-//                  and the collocation point (G depends on p - c only, so dG/dc = -sum_v dG/dp_v)
+//                  triangle's 9 vertex coordinates, its corrected area (G is linear in it) and the
+//                  collocation point: reverse mode through the pair integral of
+//                  best/triangleint.f (pair_adjoint), the three outputs' contractions at once
 //   k_adj_tri      per triangle: the self block (best/tgetgr.f:35-145), incenter, area and XMU
 //                  with their derivatives (forward mode over the 9 coordinates and the area
 //                  factor), and the direct terms of the right-hand sides and reductions
@@ -79,72 +79,6 @@ __device__ inline T tdot( const T* p, const T* q ) {
 }
 
 // ------------------------------------------------------------------ the functions differentiated
-
-// The pair block (best/triangleint.f:17-188, the formula of pair_block) as a function of the
-// source vertices, for a fixed collocation point c and corrected area ar.
-template <typename T>
-__device__ inline void pair_block_t( const T p1[ 3 ], const T p2[ 3 ], const T p3[ 3 ], const double c[ 3 ], double ar, double cg,
-                                     T g[ 6 ] ) {
-   T v21[ 3 ], v31[ 3 ], v32[ 3 ], v1c[ 3 ], v2c[ 3 ], v3c[ 3 ];
-#pragma unroll
-   for ( int k = 0; k < 3; ++k ) {
-      v21[ k ] = p2[ k ] - p1[ k ];
-      v31[ k ] = p3[ k ] - p1[ k ];
-      v32[ k ] = p3[ k ] - p2[ k ];
-      v1c[ k ] = p1[ k ] - c[ k ];
-      v2c[ k ] = p2[ k ] - c[ k ];
-      v3c[ k ] = p3[ k ] - c[ k ];
-   }
-   const T d2121 = tdot( v21, v21 ), d2131 = tdot( v21, v31 ), d2132 = tdot( v21, v32 );
-   const T d3131 = tdot( v31, v31 ), d3232 = tdot( v32, v32 ), d3231 = tdot( v32, v31 );
-   const T s21 = tsqrt( d2121 );
-   const T d211c = tdot( v21, v1c ), d212c = tdot( v21, v2c ), d322c = tdot( v32, v2c );
-   const T d1c1c = tdot( v1c, v1c ), d1c31 = tdot( v1c, v31 ), d2c2c = tdot( v2c, v2c );
-   const T d2c1c = tdot( v2c, v1c ), d3c31 = tdot( v3c, v31 ), d3c32 = tdot( v3c, v32 );
-   const T d3c3c = tdot( v3c, v3c );
-   T t0 = T {}, s0 = T {}, sx = T {}, sy = T {}, sxy = T {}, syy = T {}, sxx = T {};
-#pragma unroll 1
-   for ( int j = 0; j < 21; ++j ) {
-      const double y = c_node[ j ];
-      const double w = c_weight[ j ];
-      const T q1 = tsqrt( d1c1c + 2.0 * y * d1c31 + y * y * d3131 );
-      const T q2 = tsqrt( d2c2c + 2.0 * y * d322c + y * y * d3232 );
-      const T fh = tlog( ( d212c + d2132 * y + s21 * q2 ) / ( d211c + d2131 * y + s21 * q1 ) ) / s21;
-      T a = ( -d1c31 - y * d2131 ) / tsqrt( d1c1c + 2.0 * y * d211c + y * y * d2121 );
-      T b = ( -y * d3231 + d3c31 ) / tsqrt( y * y * d3232 - 2.0 * y * d3c32 + d3c3c );
-      T q = ( -d1c31 * d1c31 - 2.0 * y * d1c31 * d2131 - y * y * d2131 * d2131 + d1c1c * d3131 + 2.0 * y * d211c * d3131 +
-              y * y * d2121 * d3131 );
-      const T fhx = ( a + b ) / q;
-      a = ( d211c + y * d2131 ) / q1;
-      b = ( y * d2132 + d212c ) / q2;
-      q = ( d211c * d211c - d1c1c * d2121 - 2.0 * y * d1c31 * d2121 + 2.0 * y * d211c * d2131 + y * y * d2131 * d2131 -
-            y * y * d2121 * d3131 );
-      const T fhy = ( a - b ) / q;
-      b = ( y * d211c - d2c1c - 2.0 * y * d1c31 - y * d2131 - y * y * d3231 ) / q2;
-      q = ( -d211c * d211c + d1c1c * d2121 + 2.0 * y * d1c31 * d2121 - 2.0 * y * d211c * d2131 - y * y * ( d2131 * d2131 ) +
-            y * y * d2121 * d3131 );
-      const T fhxy = ( q1 + b ) / q;
-      t0 = t0 + w * fh;
-      s0 = s0 + w * fhy;
-      sx = sx + w * fhxy;
-      sy = sy + w * y * fhy;
-      sxy = sxy + w * y * fhxy;
-      syy = syy + w * y * y * fhy;
-      sxx = sxx + w * y * y * fhx;
-   }
-   t0 = 0.5 * t0, s0 = 0.5 * s0, sx = 0.5 * sx, sy = 0.5 * sy, sxy = 0.5 * sxy, syy = 0.5 * syy, sxx = 0.5 * sxx;
-   auto tab = [&]( int p, int r ) {
-      return v1c[ p ] * v1c[ r ] * s0 + ( v1c[ p ] * v21[ r ] + v21[ p ] * v1c[ r ] ) * sx +
-             ( v31[ p ] * v1c[ r ] + v1c[ p ] * v31[ r ] ) * sy + ( v21[ p ] * v31[ r ] + v31[ p ] * v21[ r ] ) * sxy +
-             v21[ p ] * v21[ r ] * sxx + v31[ p ] * v31[ r ] * syy;
-   };
-   g[ 0 ] = 2.0 * ar * ( t0 + cg * tab( 0, 0 ) );
-   g[ 1 ] = 2.0 * ar * tab( 0, 1 ) * cg;
-   g[ 2 ] = 2.0 * ar * ( t0 + cg * tab( 1, 1 ) );
-   g[ 3 ] = 2.0 * ar * tab( 0, 2 ) * cg;
-   g[ 4 ] = 2.0 * ar * tab( 1, 2 ) * cg;
-   g[ 5 ] = 2.0 * ar * ( t0 + cg * tab( 2, 2 ) );
-}
 
 template <typename T>
 __device__ inline T tf_si( T x ) {
@@ -265,6 +199,193 @@ __device__ inline void tri_sums_t( const T x[ 3 ], T ar, const TF f[ 11 ][ 3 ], 
    out[ 44 ] = ar * w4[ 2 ] * x[ 2 ];
 }
 
+// ------------------------------------------------------------------ the pair term in reverse mode
+
+// The pointer as a value the compiler cannot see through, so that loads through it are issued
+// again rather than their results held in registers (the source record is wave-uniform: SGPRs).
+__device__ inline const double* reread( const double* q ) {
+   asm volatile( "" : "+s"( q ) );
+   return q;
+}
+__host__ inline const double* reread( const double* q ) { return q; }
+
+// Reverse mode of the pair block's contractions L_o = sum_e W[o][e] g_e (o = 0, 1, 2; g the 6
+// distinct entries of best/triangleint.f's block, as pair_block writes them) with respect to the
+// source vertices p (p1, p2, p3) and the collocation point c, for a fixed corrected area ar:
+//
+//   L_o = 2 ar [ t0 tr W_o + cg ( s0 a11 + 2 sx a12 + 2 sy a13 + 2 sxy a23 + sxx a22 + syy a33 ) ],
+//
+// a_uv = u^T S_o v over the vectors v1c, v21, v31 (S_o the symmetric matrix with W_o's diagonal
+// entries on its diagonal and half its off-diagonal ones off it).  The adjoints of the seven
+// moments are known before the quadrature, so each of the 21 nodes is evaluated and then
+// differentiated at once (no tape), accumulating the adjoints of the 17 dot products; after the
+// loop they and the direct terms of tab go to the six difference vectors and to p and c.
+// gp[o][3 v + k] = dL_o / dp_v,k, gc[o][k] = dL_o / dc_k, lar[o] = L_o / ar, for NO outputs at once.
+// p (the 9 coordinates) is read from memory again after the quadrature: the six difference
+// vectors are not held in registers through it.
+template <int NO>
+__host__ __device__ inline void pair_adjoint( const double* p, const double c[ 3 ], double ar, double cg, const double ( *W )[ 6 ],
+                                              const double* node, const double* weight, double ( *gp )[ 9 ], double ( *gc )[ 3 ],
+                                              double* lar ) {
+   double v21[ 3 ], v31[ 3 ], v32[ 3 ], v1c[ 3 ], v2c[ 3 ], v3c[ 3 ];
+   auto vectors = [&]( const double* q ) {
+      for ( int k = 0; k < 3; ++k ) {
+         v21[ k ] = q[ 3 + k ] - q[ k ];
+         v31[ k ] = q[ 6 + k ] - q[ k ];
+         v32[ k ] = q[ 6 + k ] - q[ 3 + k ];
+         v1c[ k ] = q[ k ] - c[ k ];
+         v2c[ k ] = q[ 3 + k ] - c[ k ];
+         v3c[ k ] = q[ 6 + k ] - c[ k ];
+      }
+   };
+   vectors( p );
+   auto dot = []( const double* a, const double* b ) { return a[ 0 ] * b[ 0 ] + a[ 1 ] * b[ 1 ] + a[ 2 ] * b[ 2 ]; };
+   const double d2121 = dot( v21, v21 ), d2131 = dot( v21, v31 ), d2132 = dot( v21, v32 );
+   const double d3131 = dot( v31, v31 ), d3232 = dot( v32, v32 ), d3231 = dot( v32, v31 );
+   const double s21 = sqrt( d2121 ), is21 = 1.0 / s21;
+   const double d211c = dot( v21, v1c ), d212c = dot( v21, v2c ), d322c = dot( v32, v2c );
+   const double d1c1c = dot( v1c, v1c ), d1c31 = dot( v1c, v31 ), d2c2c = dot( v2c, v2c );
+   const double d2c1c = dot( v2c, v1c ), d3c31 = dot( v3c, v31 ), d3c32 = dot( v3c, v32 );
+   const double d3c3c = dot( v3c, v3c );
+
+   // S_o v for the three vectors (S_o from W_o), the quadratic forms, the moments' adjoints
+   double Sv1c[ 3 ], Sv21[ 3 ], Sv31[ 3 ];
+   auto svec = [&]( const double* w ) {
+      const double S[ 3 ][ 3 ] = { { w[ 0 ], 0.5 * w[ 1 ], 0.5 * w[ 3 ] },
+                                   { 0.5 * w[ 1 ], w[ 2 ], 0.5 * w[ 4 ] },
+                                   { 0.5 * w[ 3 ], 0.5 * w[ 4 ], w[ 5 ] } };
+      for ( int r = 0; r < 3; ++r ) {
+         Sv1c[ r ] = dot( S[ r ], v1c );
+         Sv21[ r ] = dot( S[ r ], v21 );
+         Sv31[ r ] = dot( S[ r ], v31 );
+      }
+   };
+   double bt0[ NO ], bs0[ NO ], bsx[ NO ], bsy[ NO ], bsxy[ NO ], bsxx[ NO ], bsyy[ NO ];
+   for ( int o = 0; o < NO; ++o ) {
+      const double* w = W[ o ];
+      svec( w );
+      const double a11 = dot( v1c, Sv1c ), a12 = dot( v1c, Sv21 ), a13 = dot( v1c, Sv31 );
+      const double a22 = dot( v21, Sv21 ), a23 = dot( v21, Sv31 ), a33 = dot( v31, Sv31 );
+      const double f = 2.0 * ar * cg;
+      bt0[ o ] = 2.0 * ar * ( w[ 0 ] + w[ 2 ] + w[ 5 ] );
+      bs0[ o ] = f * a11;
+      bsx[ o ] = 2.0 * f * a12;
+      bsy[ o ] = 2.0 * f * a13;
+      bsxy[ o ] = 2.0 * f * a23;
+      bsxx[ o ] = f * a22;
+      bsyy[ o ] = f * a33;
+   }
+
+   // adjoints of the dot products (and of s21), per output
+   double b2131[ NO ] = {}, b2132[ NO ] = {}, b3131[ NO ] = {}, b3232[ NO ] = {}, b3231[ NO ] = {}, bs21[ NO ] = {}, b2121[ NO ] = {};
+   double b211c[ NO ] = {}, b212c[ NO ] = {}, b322c[ NO ] = {}, b1c1c[ NO ] = {}, b1c31[ NO ] = {}, b2c2c[ NO ] = {};
+   double b2c1c[ NO ] = {}, b3c31[ NO ] = {}, b3c32[ NO ] = {}, b3c3c[ NO ] = {};
+   double t0 = 0.0, s0 = 0.0, sx = 0.0, sy = 0.0, sxy = 0.0, syy = 0.0, sxx = 0.0;
+#pragma unroll 1
+   for ( int j = 0; j < 21; ++j ) {
+      const double y = node[ j ], w = weight[ j ], yy = y * y;
+      // the node's values
+      const double A1 = d1c1c + 2.0 * y * d1c31 + yy * d3131, q1 = sqrt( A1 ), iq1 = 1.0 / q1;
+      const double A2 = d2c2c + 2.0 * y * d322c + yy * d3232, q2 = sqrt( A2 ), iq2 = 1.0 / q2;
+      const double Nn = d212c + d2132 * y + s21 * q2, Dd = d211c + d2131 * y + s21 * q1;
+      const double iNn = 1.0 / Nn, iDd = 1.0 / Dd;
+      const double fh = log( Nn * iDd ) * is21;
+      const double B1 = d1c1c + 2.0 * y * d211c + yy * d2121, ir1 = 1.0 / sqrt( B1 );
+      const double B2 = yy * d3232 - 2.0 * y * d3c32 + d3c3c, ir2 = 1.0 / sqrt( B2 );
+      const double a = ( -d1c31 - y * d2131 ) * ir1, b = ( -y * d3231 + d3c31 ) * ir2;
+      const double Q = -d1c31 * d1c31 - 2.0 * y * d1c31 * d2131 - yy * d2131 * d2131 + d1c1c * d3131 + 2.0 * y * d211c * d3131 +
+                       yy * d2121 * d3131;
+      const double iQ = 1.0 / Q, fhx = ( a + b ) * iQ;
+      const double a2 = ( d211c + y * d2131 ) * iq1, b2 = ( y * d2132 + d212c ) * iq2;
+      const double Q2 = d211c * d211c - d1c1c * d2121 - 2.0 * y * d1c31 * d2121 + 2.0 * y * d211c * d2131 + yy * d2131 * d2131 -
+                        yy * d2121 * d3131;
+      const double iQ2 = 1.0 / Q2, fhy = ( a2 - b2 ) * iQ2;
+      const double b3 = ( y * d211c - d2c1c - 2.0 * y * d1c31 - y * d2131 - yy * d3231 ) * iq2;
+      // hxy's denominator is -Q2 (the kernel spells it out; the same value to rounding)
+      const double fhxy = -( q1 + b3 ) * iQ2;
+      t0 += w * fh, s0 += w * fhy, sx += w * fhxy, sy += w * y * fhy, sxy += w * y * fhxy, syy += w * yy * fhy, sxx += w * yy * fhx;
+      const double hw = 0.5 * w;
+      for ( int o = 0; o < NO; ++o ) {
+         // seeds: the moments' adjoints times d(moment)/d(integrand) = 0.5 w y^k
+         const double gfh = hw * bt0[ o ];
+         const double gfhy = hw * ( bs0[ o ] + y * bsy[ o ] + yy * bsyy[ o ] );
+         const double gfhxy = hw * ( bsx[ o ] + y * bsxy[ o ] );
+         const double gfhx = hw * yy * bsxx[ o ];
+         // fh = (log Nn - log Dd) / s21
+         const double gNn = gfh * iNn * is21, gDd = -gfh * iDd * is21;
+         bs21[ o ] += -gfh * fh * is21 + q2 * gNn + q1 * gDd;
+         b212c[ o ] += gNn, b2132[ o ] += y * gNn;
+         b211c[ o ] += gDd, b2131[ o ] += y * gDd;
+         double gq1 = s21 * gDd, gq2 = s21 * gNn;
+         // fhx = (a + b) / Q
+         const double ga = gfhx * iQ, gb = gfhx * iQ, gQ = -gfhx * fhx * iQ;
+         b1c31[ o ] -= ga * ir1, b2131[ o ] -= y * ga * ir1;
+         const double gB1 = -0.5 * ga * a * ir1 * ir1;
+         b1c1c[ o ] += gB1, b211c[ o ] += 2.0 * y * gB1, b2121[ o ] += yy * gB1;
+         b3231[ o ] -= y * gb * ir2, b3c31[ o ] += gb * ir2;
+         const double gB2 = -0.5 * gb * b * ir2 * ir2;
+         b3232[ o ] += yy * gB2, b3c32[ o ] -= 2.0 * y * gB2, b3c3c[ o ] += gB2;
+         b1c31[ o ] += gQ * ( -2.0 * d1c31 - 2.0 * y * d2131 );
+         b2131[ o ] += gQ * ( -2.0 * y * d1c31 - 2.0 * yy * d2131 );
+         b1c1c[ o ] += gQ * d3131;
+         b3131[ o ] += gQ * B1;
+         b211c[ o ] += gQ * 2.0 * y * d3131;
+         b2121[ o ] += gQ * yy * d3131;
+         // fhy = (a2 - b2) / Q2 and fhxy = -(q1 + b3) / Q2
+         const double ga2 = gfhy * iQ2, gb2 = -gfhy * iQ2;
+         gq1 += -gfhxy * iQ2;
+         const double gb3 = -gfhxy * iQ2;
+         const double gQ2 = -( gfhy * fhy + gfhxy * fhxy ) * iQ2;
+         b211c[ o ] += ga2 * iq1, b2131[ o ] += y * ga2 * iq1, gq1 += -ga2 * a2 * iq1;
+         b2132[ o ] += y * gb2 * iq2, b212c[ o ] += gb2 * iq2, gq2 += -gb2 * b2 * iq2;
+         const double gN3 = gb3 * iq2;
+         gq2 += -gb3 * b3 * iq2;
+         b211c[ o ] += y * gN3, b2c1c[ o ] -= gN3, b1c31[ o ] -= 2.0 * y * gN3, b2131[ o ] -= y * gN3, b3231[ o ] -= yy * gN3;
+         b211c[ o ] += gQ2 * ( 2.0 * d211c + 2.0 * y * d2131 );
+         b1c1c[ o ] += gQ2 * ( -d2121 );
+         b2121[ o ] += gQ2 * ( -A1 );
+         b1c31[ o ] += gQ2 * ( -2.0 * y * d2121 );
+         b2131[ o ] += gQ2 * ( 2.0 * y * d211c + 2.0 * yy * d2131 );
+         b3131[ o ] += gQ2 * ( -yy * d2121 );
+         // q1 = sqrt(A1), q2 = sqrt(A2)
+         const double gA1 = 0.5 * gq1 * iq1, gA2 = 0.5 * gq2 * iq2;
+         b1c1c[ o ] += gA1, b1c31[ o ] += 2.0 * y * gA1, b3131[ o ] += yy * gA1;
+         b2c2c[ o ] += gA2, b322c[ o ] += 2.0 * y * gA2, b3232[ o ] += yy * gA2;
+      }
+   }
+   t0 *= 0.5, s0 *= 0.5, sx *= 0.5, sy *= 0.5, sxy *= 0.5, syy *= 0.5, sxx *= 0.5;
+
+   vectors( reread( p ) );  // the coordinates again, from memory
+   for ( int o = 0; o < NO; ++o ) {
+      const double* w = W[ o ];
+      svec( w );
+      const double a11 = dot( v1c, Sv1c ), a12 = dot( v1c, Sv21 ), a13 = dot( v1c, Sv31 );
+      const double a22 = dot( v21, Sv21 ), a23 = dot( v21, Sv31 ), a33 = dot( v31, Sv31 );
+      lar[ o ] = 2.0 * ( t0 * ( w[ 0 ] + w[ 2 ] + w[ 5 ] ) +
+                         cg * ( s0 * a11 + 2.0 * sx * a12 + 2.0 * sy * a13 + 2.0 * sxy * a23 + sxx * a22 + syy * a33 ) );
+      // the vectors' adjoints: tab's direct terms, then the dot products
+      const double f = 4.0 * ar * cg;
+      double g21[ 3 ], g31[ 3 ], g32[ 3 ], g1c[ 3 ], g2c[ 3 ], g3c[ 3 ];
+      const double bb2121 = b2121[ o ] + 0.5 * bs21[ o ] * is21;
+      for ( int k = 0; k < 3; ++k ) {
+         g1c[ k ] = f * ( s0 * Sv1c[ k ] + sx * Sv21[ k ] + sy * Sv31[ k ] );
+         g21[ k ] = f * ( sx * Sv1c[ k ] + sxy * Sv31[ k ] + sxx * Sv21[ k ] );
+         g31[ k ] = f * ( sy * Sv1c[ k ] + sxy * Sv21[ k ] + syy * Sv31[ k ] );
+         g32[ k ] = 0.0, g2c[ k ] = 0.0, g3c[ k ] = 0.0;
+         g21[ k ] += 2.0 * bb2121 * v21[ k ] + b2131[ o ] * v31[ k ] + b2132[ o ] * v32[ k ] + b211c[ o ] * v1c[ k ] + b212c[ o ] * v2c[ k ];
+         g31[ k ] += b2131[ o ] * v21[ k ] + 2.0 * b3131[ o ] * v31[ k ] + b3231[ o ] * v32[ k ] + b1c31[ o ] * v1c[ k ] + b3c31[ o ] * v3c[ k ];
+         g32[ k ] += b2132[ o ] * v21[ k ] + 2.0 * b3232[ o ] * v32[ k ] + b3231[ o ] * v31[ k ] + b322c[ o ] * v2c[ k ] + b3c32[ o ] * v3c[ k ];
+         g1c[ k ] += b211c[ o ] * v21[ k ] + 2.0 * b1c1c[ o ] * v1c[ k ] + b1c31[ o ] * v31[ k ] + b2c1c[ o ] * v2c[ k ];
+         g2c[ k ] += b212c[ o ] * v21[ k ] + b322c[ o ] * v32[ k ] + 2.0 * b2c2c[ o ] * v2c[ k ] + b2c1c[ o ] * v1c[ k ];
+         g3c[ k ] += b3c31[ o ] * v31[ k ] + b3c32[ o ] * v32[ k ] + 2.0 * b3c3c[ o ] * v3c[ k ];
+         gp[ o ][ k ] = -g21[ k ] - g31[ k ] + g1c[ k ];
+         gp[ o ][ 3 + k ] = g21[ k ] - g32[ k ] + g2c[ k ];
+         gp[ o ][ 6 + k ] = g31[ k ] + g32[ k ] + g3c[ k ];
+         gc[ o ][ k ] = -( g1c[ k ] + g2c[ k ] + g3c[ k ] );
+      }
+   }
+}
+
 // ------------------------------------------------------------------ kernels
 
 // y_o(3 i + c, k) = sum_s ghat[o][s] d(sum_s)/dF(3 i + c, k): the reductions are linear in F,
@@ -316,31 +437,37 @@ __global__ void __launch_bounds__( 256 ) k_transpose( int nt, T* __restrict__ a,
 }
 
 // lambda_o,k at triangle i component a: lam[o * 11 * ld + k * ld + 3 i + a]; F likewise without o.
-// Per pair (n1 collocation = lane, n2 source = uniform over the wave): M_o = sum_k lambda_o,k,n1
-// F_k,n2^T, and for each of the source's 9 coordinates theta the contraction sum_ab M_o(a, b)
-// dG(a, b)/dtheta; the same with the sign flipped for the collocation point; and M_o : G / area.
-// gsrc: per (collocation block, n2) the 30 sums over the block's lanes (o-major: 9 coordinates,
-// then the area); gcol: per (source strip, n1) 9 values (o-major, 3 coordinates).
-// Two waves per SIMD although the registers then spill (93 VGPRs): the second wave hides the
-// latency of the pair integrals' transcendental chains, 133 -> 117 ms at 5,120 triangles.
+// Per pair (n1 collocation = lane, n2 source = uniform over the wave) and output o: M_o =
+// sum_k lambda_o,k,n1 F_k,n2^T, contracted with the pair block G (symmetric: 6 weights), and the
+// gradient of that contraction with respect to the source's 9 coordinates and the collocation
+// point by pair_adjoint, plus M_o : G / area.  Every lane evaluates (so the wave sums need no
+// divergent branch): on the diagonal pair and on idle lanes the weights are 0 and the point is
+// moved off the triangle, so their terms are exact zeros.  gsrc: per (collocation block, n2) the
+// 30 sums over the block's lanes (o-major: 9 coordinates, then the area); gcol: per (source
+// strip, n1) 9 values (o-major, 3 coordinates).
 __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( 2 ) ) ) k_adj_pairs( int n, int strip, double cg, const double* __restrict__ src,
                                                      const double* __restrict__ xc, const double* __restrict__ lam,
                                                      const double* __restrict__ F, long ld, double* __restrict__ gsrc,
                                                      double* __restrict__ gcol ) {
-   __shared__ double red[ 4 ][ 32 ];
+   __shared__ double red[ 4 ][ kAdjOutputs * 10 ];
    const int n1 = blockIdx.x * blockDim.x + threadIdx.x;
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int n2lo = blockIdx.y * strip, n2hi = min( n, n2lo + strip );
    const bool act = n1 < n;
    const int m1 = act ? n1 : 0;
    const double c[ 3 ] = { xc[ m1 ], xc[ n + m1 ], xc[ 2 * n + m1 ] };
-   double gc[ kAdjOutputs ][ 3 ] = {};
+   double gcs[ kAdjOutputs ][ 3 ] = {};
    for ( int n2 = n2lo; n2 < n2hi; ++n2 ) {
       const double* s = src + (size_t)kSrcStride * n2;
       const bool live = act && n2 != n1;
-      // M_o (symmetric part: G is symmetric), the contraction weights of the 6 distinct entries
-      double wsym[ kAdjOutputs ][ 6 ];
+      const double* p = s + kP1;
+      // a point off the source triangle for the lanes whose terms are zero
+      const double off = live ? 0.0 : 4.0 * ( fabs( p[ 3 ] - p[ 0 ] ) + fabs( p[ 6 ] - p[ 0 ] ) + fabs( p[ 4 ] - p[ 1 ] ) +
+                                              fabs( p[ 7 ] - p[ 1 ] ) + 1.0 );
+      const double ce[ 3 ] = { live ? c[ 0 ] : p[ 0 ] + off, live ? c[ 1 ] : p[ 1 ] + off, live ? c[ 2 ] : p[ 2 ] + off };
+#pragma unroll 1
       for ( int o = 0; o < kAdjOutputs; ++o ) {
+         // M_o (symmetric part: G is symmetric), the contraction weights of the 6 distinct entries
          double M[ 3 ][ 3 ] = {};
          for ( int k = 0; k < 11; ++k ) {
             const double* l = lam + (long)o * 11 * ld + (long)k * ld + 3L * m1;
@@ -350,59 +477,21 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( 2
             M[ 1 ][ 0 ] += l1 * f0, M[ 1 ][ 1 ] += l1 * f1, M[ 1 ][ 2 ] += l1 * f2;
             M[ 2 ][ 0 ] += l2 * f0, M[ 2 ][ 1 ] += l2 * f1, M[ 2 ][ 2 ] += l2 * f2;
          }
-         wsym[ o ][ 0 ] = M[ 0 ][ 0 ];
-         wsym[ o ][ 1 ] = M[ 1 ][ 0 ] + M[ 0 ][ 1 ];
-         wsym[ o ][ 2 ] = M[ 1 ][ 1 ];
-         wsym[ o ][ 3 ] = M[ 2 ][ 0 ] + M[ 0 ][ 2 ];
-         wsym[ o ][ 4 ] = M[ 2 ][ 1 ] + M[ 1 ][ 2 ];
-         wsym[ o ][ 5 ] = M[ 2 ][ 2 ];
-      }
-      // The pair is singular on the diagonal (and meaningless on idle lanes): its terms are 0 there.
-      // Each term is summed over the wave as soon as it exists (into red, the same fixed order as
-      // a sum after the loop), so no per-direction array stays live across the nine directions.
-      auto wave_store = [&]( int q, const double ( &t )[ kAdjOutputs ] ) {
-         for ( int o = 0; o < kAdjOutputs; ++o ) {
-            double v = t[ o ];
+         const double z = live ? 1.0 : 0.0;
+         const double W[ 1 ][ 6 ] = { { z * M[ 0 ][ 0 ], z * ( M[ 1 ][ 0 ] + M[ 0 ][ 1 ] ), z * M[ 1 ][ 1 ],
+                                        z * ( M[ 2 ][ 0 ] + M[ 0 ][ 2 ] ), z * ( M[ 2 ][ 1 ] + M[ 1 ][ 2 ] ), z * M[ 2 ][ 2 ] } };
+         double gp[ 1 ][ 9 ], gc[ 1 ][ 3 ], lar[ 1 ];
+         pair_adjoint<1>( p, ce, s[ kArea ], cg, W, c_node, c_weight, gp, gc, lar );
+         for ( int k = 0; k < 3; ++k )  // selects: o is not a compile-time index here
+            for ( int oo = 0; oo < kAdjOutputs; ++oo ) gcs[ oo ][ k ] += oo == o ? gc[ 0 ][ k ] : 0.0;
+         // the 10 terms summed over the wave (a fixed order)
 #pragma unroll
-            for ( int off = 32; off > 0; off >>= 1 ) v += __shfl_xor( v, off );
-            v = __shfl( v, 0 );
+         for ( int q = 0; q < 10; ++q ) {
+            double v = q < 9 ? gp[ 0 ][ q ] : lar[ 0 ];
+#pragma unroll
+            for ( int sh = 32; sh > 0; sh >>= 1 ) v += __shfl_xor( v, sh );
             if ( lane == 0 ) red[ wave ][ o * 10 + q ] = v;
          }
-      };
-      const double ar = s[ kArea ];
-      {
-         double t[ kAdjOutputs ] = {};
-         if ( live ) {
-            double g6[ 6 ];
-            pair_block( s, c, cg, g6 );
-            for ( int o = 0; o < kAdjOutputs; ++o ) {
-               double u = 0.0;
-               for ( int e = 0; e < 6; ++e ) u += wsym[ o ][ e ] * g6[ e ];
-               t[ o ] = u / ar;
-            }
-         }
-         wave_store( 9, t );
-      }
-      for ( int dir = 0; dir < 9; ++dir ) {
-         double t[ kAdjOutputs ] = {};
-         if ( live ) {
-            D1 p[ 3 ][ 3 ];
-            for ( int v = 0; v < 3; ++v )
-               for ( int k = 0; k < 3; ++k ) p[ v ][ k ] = mk( s[ kP1 + 3 * v + k ], dir == 3 * v + k ? 1.0 : 0.0 );
-            D1 g[ 6 ];
-            pair_block_t<D1>( p[ 0 ], p[ 1 ], p[ 2 ], c, ar, cg, g );
-            const int k3 = dir % 3;
-            for ( int o = 0; o < kAdjOutputs; ++o ) {
-               double u = 0.0;
-               for ( int e = 0; e < 6; ++e ) u += wsym[ o ][ e ] * g[ e ].d;
-               t[ o ] = u;
-               // the collocation point moves opposite to the source: selects, not a lane-varying index
-               gc[ o ][ 0 ] -= k3 == 0 ? u : 0.0;
-               gc[ o ][ 1 ] -= k3 == 1 ? u : 0.0;
-               gc[ o ][ 2 ] -= k3 == 2 ? u : 0.0;
-            }
-         }
-         wave_store( dir, t );
       }
       __syncthreads();
       if ( threadIdx.x < kAdjOutputs * 10 )
@@ -412,7 +501,7 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( 2
    }
    if ( act )
       for ( int o = 0; o < kAdjOutputs; ++o )
-         for ( int k = 0; k < 3; ++k ) gcol[ ( (long)blockIdx.y * n + n1 ) * ( kAdjOutputs * 3 ) + o * 3 + k ] = gc[ o ][ k ];
+         for ( int k = 0; k < 3; ++k ) gcol[ ( (long)blockIdx.y * n + n1 ) * ( kAdjOutputs * 3 ) + o * 3 + k ] = gcs[ o ][ k ];
 }
 
 // out[i * width + q] = sum over the nparts partials in order.
