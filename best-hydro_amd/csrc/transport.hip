@@ -466,21 +466,26 @@ void jacobi3( const double* a, double d[ 3 ] ) {
 }
 
 // The kSumSlots sums from the workgroups' partials, each added in workgroup order (the order
-// the host used), on one thread.
+// the host used), one slot per thread of the workgroup, into LDS.
 __device__ void total_sums( const double* __restrict__ partial, double* s ) {
-   for ( int k = 0; k < kSumSlots; ++k ) {
+   const int k = threadIdx.x;
+   if ( k < kSumSlots ) {
       double t = 0.0;
       for ( int g = 0; g < kSumGroups; ++g ) t = t + partial[ (long)g * kSumSlots + k ];
       s[ k ] = t;
    }
+   __syncthreads();
 }
 
 // DIFCO, VISCO (at the centroid, or the centre of viscosity when icov: the chiprime then comes
-// from k_transport_chi after the sums at that centre) and OUTPUT on one thread.
-__global__ void k_transport( const double* __restrict__ partial, int ivisco, int icov, double cg, Transport* out ) {
-   if ( blockIdx.x || threadIdx.x ) return;
+// from k_transport_chi after the sums at that centre) and OUTPUT: the sums on 64 threads, the
+// algebra on one.
+__global__ void __launch_bounds__( 64 ) k_transport( const double* __restrict__ partial, int ivisco, int icov, double cg, Transport* out ) {
+   __shared__ double sh[ kSumSlots ];
+   total_sums( partial, sh );
+   if ( threadIdx.x ) return;
    double s[ kSumSlots ];
-   total_sums( partial, s );
+   for ( int k = 0; k < kSumSlots; ++k ) s[ k ] = sh[ k ];
    Transport T {};
    difco_algebra( s, cg, T );
    T.chiprime = 0.0;
@@ -492,10 +497,12 @@ __global__ void k_transport( const double* __restrict__ partial, int ivisco, int
    *out = T;
 }
 
-__global__ void k_transport_chi( const double* __restrict__ partial, double cg, Transport* out ) {
-   if ( blockIdx.x || threadIdx.x ) return;
+__global__ void __launch_bounds__( 64 ) k_transport_chi( const double* __restrict__ partial, double cg, Transport* out ) {
+   __shared__ double sh[ kSumSlots ];
+   total_sums( partial, sh );
+   if ( threadIdx.x ) return;
    double s[ kSumSlots ];
-   total_sums( partial, s );
+   for ( int k = 0; k < kSumSlots; ++k ) s[ k ] = sh[ k ];
    out->chiprime = visco_chiprime( s, cg );
 }
 
@@ -503,12 +510,12 @@ void launch_transport( hipStream_t st, int n, const double* xc, const double* ar
                        int icov, double cg, double* partial, Transport* d_out ) {
    hipLaunchKernelGGL( k_sums, dim3( kSumGroups ), dim3( 256 ), 0, st, n, xc, area, b, ldb, ivisco ? 5 : 0, 0.0, 0.0, 0.0,
                        nullptr, icov ? 1 : 0, partial );
-   hipLaunchKernelGGL( k_transport, dim3( 1 ), dim3( 1 ), 0, st, partial, ivisco, icov, cg, d_out );
+   hipLaunchKernelGGL( k_transport, dim3( 1 ), dim3( 64 ), 0, st, partial, ivisco, icov, cg, d_out );
    if ( ivisco && icov ) {
       const double* d_centre = reinterpret_cast<const double*>( reinterpret_cast<const char*>( d_out ) + offsetof( Transport, dvisc ) );
       hipLaunchKernelGGL( k_sums, dim3( kSumGroups ), dim3( 256 ), 0, st, n, xc, area, b, ldb, 5, 0.0, 0.0, 0.0, d_centre, 0,
                           partial );
-      hipLaunchKernelGGL( k_transport_chi, dim3( 1 ), dim3( 1 ), 0, st, partial, cg, d_out );
+      hipLaunchKernelGGL( k_transport_chi, dim3( 1 ), dim3( 64 ), 0, st, partial, cg, d_out );
    }
 }
 
