@@ -4,13 +4,16 @@ backward is the adjoint gradient of ``bhg_gradient`` (include/best_hip.h).
 
 The surfaces (COMPON's ``icomp``, best/compon.f90:19-115) are taken from the unperturbed mesh and
 held fixed, so a vertex moved off its neighbours' copies does not change the deflation.
+
+torch is imported only when ``transport`` is first called: ``MeshTransport`` (values and adjoint
+gradients as numpy arrays) does not load it, so a process that only solves never carries torch's
+own HIP libraries beside the library's.
 """
 from __future__ import annotations
 
 import ctypes as C
 
 import numpy as np
-import torch
 
 from . import Solver, lib, meshes, _col
 
@@ -62,21 +65,33 @@ class MeshTransport:
         return np.array([r[k] for k in keys])
 
 
-class TransportFunction(torch.autograd.Function):
-    """forward(V, mt) -> (D_t, D_r, CHI V[, eta]) as a float64 tensor; backward by the adjoint."""
-
-    @staticmethod
-    def forward(ctx, V, mt: MeshTransport):
-        vals, grad = mt.values_and_gradient(V.detach().cpu().numpy().astype(np.float64))
-        ctx.save_for_backward(torch.from_numpy(grad))
-        return torch.from_numpy(vals).to(V.device)
-
-    @staticmethod
-    def backward(ctx, gout):
-        (grad,) = ctx.saved_tensors
-        gv = torch.einsum("o,ovc->vc", gout.detach().cpu().to(torch.float64), grad)
-        return gv.to(gout.device), None
+_FUNCTION = None
 
 
-def transport(V: torch.Tensor, mt: MeshTransport) -> torch.Tensor:
-    return TransportFunction.apply(V, mt)
+def transport_function():
+    """The torch.autograd.Function: forward(V, mt) -> (D_t, D_r, CHI V[, eta]) as a float64 tensor,
+    backward by the adjoint (built on first use, with torch imported then)."""
+    global _FUNCTION
+    if _FUNCTION is None:
+        import torch
+
+        class TransportFunction(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, V, mt: MeshTransport):
+                vals, grad = mt.values_and_gradient(V.detach().cpu().numpy().astype(np.float64))
+                ctx.save_for_backward(torch.from_numpy(grad))
+                return torch.from_numpy(vals).to(V.device)
+
+            @staticmethod
+            def backward(ctx, gout):
+                (grad,) = ctx.saved_tensors
+                gv = torch.einsum("o,ovc->vc", gout.detach().cpu().to(torch.float64), grad)
+                return gv.to(gout.device), None
+
+        _FUNCTION = TransportFunction
+    return _FUNCTION
+
+
+def transport(V, mt: MeshTransport):
+    """(D_t, D_r, CHI V[, eta]) of the vertices V (a torch tensor), differentiable in V."""
+    return transport_function().apply(V, mt)

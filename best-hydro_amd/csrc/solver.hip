@@ -742,39 +742,9 @@ void solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int 
 
 // ------------------------------------------------------------------ adjoint
 
-// The differentiated outputs from the reductions (slots 0..44, no centre-of-viscosity move):
-// D_t and D_r as output_values prints them, CHI * V.
-void adjoint_outputs( const double* sums, double cg, double q[ kAdjOutputs ] ) {
-   Transport tr {};
-   difco_algebra( sums, cg, tr );
-   output_values( tr );
-   q[ 0 ] = tr.dtrans;
-   q[ 1 ] = tr.dr;
-   q[ 2 ] = visco_chiprime( sums, cg );
-}
-
-// ghat[o][q] = d(output o)/d(slot q) by fourth-order central differences of the host algebra (a
-// few hundred 3x3 evaluations).  The step is 1e-4 of the largest magnitude among the slots of
-// the same kind (force or torque of a translation or a rotation, the CHI moments), so that slots
-// that vanish by symmetry get a step of the right scale.
-void adjoint_weights( const double* sums, double cg, double* ghat ) {
-   auto kind = []( int q ) { return q < 36 ? 2 * ( q / 6 >= 3 ) + ( q % 6 >= 3 ) : 4; };
-   double scale[ 5 ] = {};
-   for ( int q = 0; q < kAdjSlots; ++q ) scale[ kind( q ) ] = std::max( scale[ kind( q ) ], std::fabs( sums[ q ] ) );
-   std::vector<double> t( sums, sums + kSumSlots );
-   for ( int q = 0; q < kAdjSlots; ++q ) {
-      const double h = 1e-4 * ( scale[ kind( q ) ] > 0 ? scale[ kind( q ) ] : 1.0 );
-      double f[ 4 ][ kAdjOutputs ];
-      const double off[ 4 ] = { -2 * h, -h, h, 2 * h };
-      for ( int e = 0; e < 4; ++e ) {
-         t[ q ] = sums[ q ] + off[ e ];
-         adjoint_outputs( t.data(), cg, f[ e ] );
-      }
-      t[ q ] = sums[ q ];
-      for ( int o = 0; o < kAdjOutputs; ++o )
-         ghat[ o * kAdjSlots + q ] = ( f[ 0 ][ o ] - 8.0 * f[ 1 ][ o ] + 8.0 * f[ 2 ][ o ] - f[ 3 ][ o ] ) / ( 12.0 * h );
-   }
-}
+// ghat[o][q] = d(output o)/d(slot q), slots 0..44 (DIFCO's forces and torques, the CHI moments;
+// no centre-of-viscosity move): exact derivatives of the algebra (algebra_gradient, forward mode).
+void adjoint_weights( const double* sums, double cg, double* ghat ) { algebra_gradient( sums, cg, kAdjSlots, ghat ); }
 
 template <typename T>
 void swap_bufs( DevBuf<T>& a, DevBuf<T>& b ) {

@@ -27,6 +27,9 @@ __host__ __device__ void visco_centre( const double* sums, double d[ 3 ] );
 __host__ __device__ double visco_chiprime( const double* sums, double cg );
 __host__ __device__ void output_values( Transport& T );
 void jacobi3( const double* a, double d[ 3 ] );  // host
+// d(D_t, D_r, CHI * V) / d(sum q), q < nslots, exactly (forward mode through the algebra; host):
+// ghat[o * nslots + q].
+void algebra_gradient( const double* sums, double cg, int nslots, double* ghat );
 // The reductions and the whole algebra on the device, into d_out (device memory): partial holds
 // kSumGroups x kSumSlots doubles.
 void launch_transport( hipStream_t st, int n, const double* xc, const double* area, const double* b, long ldb, int ivisco,

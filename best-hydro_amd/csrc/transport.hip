@@ -177,45 +177,71 @@ namespace {
 
 #define BH_HD __host__ __device__
 
+// A value and one derivative (forward mode), for the exact derivatives of the algebra below.
+struct Dual {
+   double v, d;
+   BH_HD Dual( double x = 0.0, double dx = 0.0 ) : v( x ), d( dx ) {}
+};
+BH_HD inline Dual operator+( Dual a, Dual b ) { return Dual( a.v + b.v, a.d + b.d ); }
+BH_HD inline Dual operator-( Dual a, Dual b ) { return Dual( a.v - b.v, a.d - b.d ); }
+BH_HD inline Dual operator-( Dual a ) { return Dual( -a.v, -a.d ); }
+BH_HD inline Dual operator*( Dual a, Dual b ) { return Dual( a.v * b.v, a.d * b.v + a.v * b.d ); }
+BH_HD inline Dual operator/( Dual a, Dual b ) {
+   const double q = a.v / b.v;
+   return Dual( q, ( a.d - q * b.d ) / b.v );
+}
+BH_HD inline double val( double x ) { return x; }
+BH_HD inline double val( Dual x ) { return x.v; }
+BH_HD inline double tabs( double x ) { return fabs( x ); }
+BH_HD inline Dual tabs( Dual x ) { return x.v < 0.0 ? -x : x; }
+
 // 3x3 matrices column-major: m[i + 3 j] is (i, j), 0-based.
-BH_HD inline double& at( double* m, int i, int j ) { return m[ i + 3 * j ]; }
-BH_HD inline double at( const double* m, int i, int j ) { return m[ i + 3 * j ]; }
-BH_HD inline void copy9( double* d, const double* s ) {
+template <typename T>
+BH_HD inline T& at( T* m, int i, int j ) {
+   return m[ i + 3 * j ];
+}
+template <typename T>
+BH_HD inline T at( const T* m, int i, int j ) {
+   return m[ i + 3 * j ];
+}
+template <typename T>
+BH_HD inline void copy9( T* d, const T* s ) {
    for ( int k = 0; k < 9; ++k ) d[ k ] = s[ k ];
 }
 
 // The inverse of a 3x3 matrix by LU with implicit (scaled) partial pivoting and back
 // substitution per unit column (best/matin3.f).  a is overwritten.  Returns nonzero when a row is
-// all zeros.
-BH_HD int inverse3( double* a, double* y ) {
+// all zeros.  The pivots are chosen on the values (T = Dual carries the derivatives along).
+template <typename T>
+BH_HD int inverse3( T* a, T* y ) {
    for ( int i = 0; i < 3; ++i )
-      for ( int j = 0; j < 3; ++j ) at( y, i, j ) = i == j ? 1.0 : 0.0;
+      for ( int j = 0; j < 3; ++j ) at( y, i, j ) = T( i == j ? 1.0 : 0.0 );
    double scale[ 3 ];
    for ( int i = 0; i < 3; ++i ) {
       double big = 0.0;
-      for ( int j = 0; j < 3; ++j ) big = fabs( at( a, i, j ) ) > big ? fabs( at( a, i, j ) ) : big;
+      for ( int j = 0; j < 3; ++j ) big = fabs( val( at( a, i, j ) ) ) > big ? fabs( val( at( a, i, j ) ) ) : big;
       if ( big == 0.0 ) return 1;
       scale[ i ] = 1.0 / big;
    }
    int perm[ 3 ];
    for ( int j = 0; j < 3; ++j ) {
       for ( int i = 1; i < j; ++i ) {
-         double s = at( a, i, j );
+         T s = at( a, i, j );
          for ( int k = 0; k < i; ++k ) s = s - at( a, i, k ) * at( a, k, j );
          at( a, i, j ) = s;
       }
       double big = 0.0;
       int imax = j;
       for ( int i = j; i < 3; ++i ) {
-         double s = at( a, i, j );
+         T s = at( a, i, j );
          for ( int k = 0; k < j; ++k ) s = s - at( a, i, k ) * at( a, k, j );
          if ( j > 0 ) at( a, i, j ) = s;
-         const double r = scale[ i ] * fabs( s );
+         const double r = scale[ i ] * fabs( val( s ) );
          if ( r >= big ) big = r, imax = i;
       }
       if ( imax != j ) {
          for ( int k = 0; k < 3; ++k ) {
-            const double t = at( a, imax, k );
+            const T t = at( a, imax, k );
             at( a, imax, k ) = at( a, j, k );
             at( a, j, k ) = t;
          }
@@ -223,27 +249,27 @@ BH_HD int inverse3( double* a, double* y ) {
       }
       perm[ j ] = imax;
       if ( j != 2 ) {
-         if ( at( a, j, j ) == 0.0 ) at( a, j, j ) = 1e-20;
-         const double r = 1.0 / at( a, j, j );
+         if ( val( at( a, j, j ) ) == 0.0 ) at( a, j, j ) = T( 1e-20 );
+         const T r = T( 1.0 ) / at( a, j, j );
          for ( int i = j + 1; i < 3; ++i ) at( a, i, j ) = at( a, i, j ) * r;
       }
    }
-   if ( at( a, 2, 2 ) == 0.0 ) at( a, 2, 2 ) = 1e-20;
+   if ( val( at( a, 2, 2 ) ) == 0.0 ) at( a, 2, 2 ) = T( 1e-20 );
    for ( int c = 0; c < 3; ++c ) {
-      double* v = y + 3 * c;
+      T* v = y + 3 * c;
       int first = 0;
       for ( int i = 0; i < 3; ++i ) {
-         double s = v[ perm[ i ] ];
+         T s = v[ perm[ i ] ];
          v[ perm[ i ] ] = v[ i ];
          if ( first != 0 ) {
             for ( int j = first - 1; j < i; ++j ) s = s - at( a, i, j ) * v[ j ];
-         } else if ( s != 0.0 ) {
+         } else if ( val( s ) != 0.0 ) {
             first = 1;
          }
          v[ i ] = s;
       }
       for ( int i = 2; i >= 0; --i ) {
-         double s = v[ i ];
+         T s = v[ i ];
          for ( int j = i + 1; j < 3; ++j ) s = s - at( a, i, j ) * v[ j ];
          v[ i ] = s / at( a, i, i );
       }
@@ -299,29 +325,37 @@ BH_HD void eigen3( double* a, double d[ 3 ] ) {
 
 // uu = the matrix of the centre vector r of best/difco.f:180-188 (the cross-product matrix,
 // with r(1) at (3,2)).
-BH_HD void cross_matrix( const double r[ 3 ], double* uu ) {
+template <typename T>
+BH_HD void cross_matrix( const T r[ 3 ], T* uu ) {
    at( uu, 2, 0 ) = -r[ 1 ];
    at( uu, 2, 1 ) = r[ 0 ];
    at( uu, 0, 1 ) = -r[ 2 ];
    at( uu, 1, 0 ) = -at( uu, 0, 1 );
    at( uu, 1, 2 ) = -at( uu, 2, 1 );
    at( uu, 0, 2 ) = -at( uu, 2, 0 );
-   at( uu, 0, 0 ) = at( uu, 1, 1 ) = at( uu, 2, 2 ) = 0.0;
+   at( uu, 0, 0 ) = at( uu, 1, 1 ) = at( uu, 2, 2 ) = T( 0.0 );
 }
 
 }  // namespace
 
-BH_HD void difco_algebra( const double* s, double cg, Transport& T ) {
-   double ktt[ 9 ], ktr[ 9 ], krt[ 9 ], krr[ 9 ], uu[ 9 ], aux[ 9 ], kinv[ 9 ], work[ 9 ];
+// DIFCO's algebra (best/difco.f:106-401) on the 36 force and torque sums, into the tensors of o.
+template <typename R>
+struct DifcoOut {
+   R dtt[ 9 ], dtr[ 9 ], drr[ 9 ], urc[ 3 ], udr[ 3 ], ktt[ 9 ], ktr[ 9 ], krt[ 9 ], krr[ 9 ];
+};
+
+template <typename R>
+BH_HD void difco_t( const R* s, double cg, DifcoOut<R>& o ) {
+   R ktt[ 9 ], ktr[ 9 ], krt[ 9 ], krr[ 9 ], uu[ 9 ], aux[ 9 ], kinv[ 9 ], work[ 9 ];
    // Friction tensors: column i of Ktt/Ktr from translation i, of Krt/Krr from rotation i.
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
-         at( ktt, j, i ) = -s[ 6 * i + j ] * ( 1.0 / 1.0 );
-         at( ktr, j, i ) = -s[ 6 * i + 3 + j ] * ( 1.0 / 1.0 );
-         at( krt, j, i ) = -s[ 6 * ( 3 + i ) + j ] * ( 1.0 / 1.0 );
-         at( krr, j, i ) = -s[ 6 * ( 3 + i ) + 3 + j ] * ( 1.0 / 1.0 );
+         at( ktt, j, i ) = -s[ 6 * i + j ] * R( 1.0 / 1.0 );
+         at( ktr, j, i ) = -s[ 6 * i + 3 + j ] * R( 1.0 / 1.0 );
+         at( krt, j, i ) = -s[ 6 * ( 3 + i ) + j ] * R( 1.0 / 1.0 );
+         at( krr, j, i ) = -s[ 6 * ( 3 + i ) + 3 + j ] * R( 1.0 / 1.0 );
       }
-   copy9( T.ktr, ktr );
+   copy9( o.ktr, ktr );
    // Centre of resistance (difco.f:141-174).
    at( uu, 0, 0 ) = at( ktt, 1, 1 ) + at( ktt, 2, 2 );
    at( uu, 1, 1 ) = at( ktt, 0, 0 ) + at( ktt, 2, 2 );
@@ -333,60 +367,60 @@ BH_HD void difco_algebra( const double* s, double cg, Transport& T ) {
    at( uu, 1, 2 ) = at( uu, 2, 1 );
    at( uu, 2, 0 ) = at( uu, 0, 2 );
    for ( int i = 0; i < 3; ++i )
-      for ( int j = 0; j < 3; ++j ) at( krt, i, j ) = ( at( krt, i, j ) + at( ktr, j, i ) ) * 0.5;
-   const double sur1 = -at( krt, 2, 1 ) + at( krt, 1, 2 );
-   const double sur2 = -at( krt, 0, 2 ) + at( krt, 2, 0 );
-   const double sur3 = -at( krt, 1, 0 ) + at( krt, 0, 1 );
+      for ( int j = 0; j < 3; ++j ) at( krt, i, j ) = ( at( krt, i, j ) + at( ktr, j, i ) ) * R( 0.5 );
+   const R sur1 = -at( krt, 2, 1 ) + at( krt, 1, 2 );
+   const R sur2 = -at( krt, 0, 2 ) + at( krt, 2, 0 );
+   const R sur3 = -at( krt, 1, 0 ) + at( krt, 0, 1 );
    inverse3( uu, aux );
-   for ( int j = 0; j < 3; ++j ) T.urc[ j ] = ( at( aux, j, 0 ) * sur1 + at( aux, j, 1 ) * sur2 + at( aux, j, 2 ) * sur3 );
-   cross_matrix( T.urc, uu );
+   for ( int j = 0; j < 3; ++j ) o.urc[ j ] = ( at( aux, j, 0 ) * sur1 + at( aux, j, 1 ) * sur2 + at( aux, j, 2 ) * sur3 );
+   cross_matrix( o.urc, uu );
    // Krr' = Krr - uu Ktt uu + Krt^T uu - uu Krt,  Krt' = Krt + Ktt uu  (difco.f:196-214).
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
-         double t = 0.0;
+         R t = R( 0.0 );
          for ( int k = 0; k < 3; ++k )
             t = t + at( uu, i, k ) * ( at( ktt, k, 0 ) * at( uu, 0, j ) + at( ktt, k, 1 ) * at( uu, 1, j ) + at( ktt, k, 2 ) * at( uu, 2, j ) );
-         const double t1 = at( krt, 0, i ) * at( uu, 0, j ) + at( krt, 1, i ) * at( uu, 1, j ) + at( krt, 2, i ) * at( uu, 2, j );
-         const double t2 = at( uu, i, 0 ) * at( krt, 0, j ) + at( uu, i, 1 ) * at( krt, 1, j ) + at( uu, i, 2 ) * at( krt, 2, j );
+         const R t1 = at( krt, 0, i ) * at( uu, 0, j ) + at( krt, 1, i ) * at( uu, 1, j ) + at( krt, 2, i ) * at( uu, 2, j );
+         const R t2 = at( uu, i, 0 ) * at( krt, 0, j ) + at( uu, i, 1 ) * at( krt, 1, j ) + at( uu, i, 2 ) * at( krt, 2, j );
          at( krr, i, j ) = at( krr, i, j ) - t + t1 - t2;
       }
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j )
          at( krt, i, j ) = at( krt, i, j ) + at( ktt, i, 0 ) * at( uu, 0, j ) + at( ktt, i, 1 ) * at( uu, 1, j ) + at( ktt, i, 2 ) * at( uu, 2, j );
-   copy9( T.ktt, ktt );
-   copy9( T.krt, krt );
-   copy9( T.krr, krr );
+   copy9( o.ktt, ktt );
+   copy9( o.krt, krt );
+   copy9( o.krr, krr );
    // Dtt, Dtr, Drr with the coupling (difco.f:244-330).
    copy9( work, ktt );
    inverse3( work, kinv );
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
-         double t = 0.0;
+         R t = R( 0.0 );
          for ( int k = 0; k < 3; ++k )
             t = t + at( krt, i, k ) * ( at( kinv, k, 0 ) * at( krt, j, 0 ) + at( kinv, k, 1 ) * at( krt, j, 1 ) + at( kinv, k, 2 ) * at( krt, j, 2 ) );
          at( work, i, j ) = at( krr, i, j ) - t;
       }
-   inverse3( work, T.drr );
+   inverse3( work, o.drr );
    if ( cg == 0 )
-      for ( int k = 0; k < 9; ++k ) T.drr[ k ] = 4.0 * T.drr[ k ] / 3.0;
+      for ( int k = 0; k < 9; ++k ) o.drr[ k ] = R( 4.0 ) * o.drr[ k ] / R( 3.0 );
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
-         double t = 0.0;
+         R t = R( 0.0 );
          for ( int k = 0; k < 3; ++k )
-            t = t + at( kinv, i, k ) * ( at( krt, 0, k ) * at( T.drr, 0, j ) + at( krt, 1, k ) * at( T.drr, 1, j ) + at( krt, 2, k ) * at( T.drr, 2, j ) );
-         at( T.dtr, i, j ) = -t;
-         if ( cg == 0 ) at( T.dtr, i, j ) = 4.0 * at( T.dtr, i, j ) / 3.0;
+            t = t + at( kinv, i, k ) * ( at( krt, 0, k ) * at( o.drr, 0, j ) + at( krt, 1, k ) * at( o.drr, 1, j ) + at( krt, 2, k ) * at( o.drr, 2, j ) );
+         at( o.dtr, i, j ) = -t;
+         if ( cg == 0 ) at( o.dtr, i, j ) = R( 4.0 ) * at( o.dtr, i, j ) / R( 3.0 );
       }
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
-         double t = 0.0;
+         R t = R( 0.0 );
          for ( int k = 0; k < 3; ++k )
-            t = t + at( kinv, i, k ) * ( at( krt, 0, k ) * at( T.dtr, j, 0 ) + at( krt, 1, k ) * at( T.dtr, j, 1 ) + at( krt, 2, k ) * at( T.dtr, j, 2 ) );
-         at( T.dtt, i, j ) = at( kinv, i, j ) - t;
-         if ( cg == 0 ) at( T.dtt, i, j ) = 4.0 * at( T.dtt, i, j ) / 3.0;
+            t = t + at( kinv, i, k ) * ( at( krt, 0, k ) * at( o.dtr, j, 0 ) + at( krt, 1, k ) * at( o.dtr, j, 1 ) + at( krt, 2, k ) * at( o.dtr, j, 2 ) );
+         at( o.dtt, i, j ) = at( kinv, i, j ) - t;
+         if ( cg == 0 ) at( o.dtt, i, j ) = R( 4.0 ) * at( o.dtt, i, j ) / R( 3.0 );
       }
    // Centre of diffusion (difco.f:344-401).
-   const double* drr = T.drr;
+   const R* drr = o.drr;
    at( uu, 0, 0 ) = at( drr, 1, 1 ) + at( drr, 2, 2 );
    at( uu, 1, 1 ) = at( drr, 0, 0 ) + at( drr, 2, 2 );
    at( uu, 2, 2 ) = at( drr, 1, 1 ) + at( drr, 0, 0 );
@@ -396,26 +430,34 @@ BH_HD void difco_algebra( const double* s, double cg, Transport& T ) {
    at( uu, 1, 0 ) = at( uu, 0, 1 );
    at( uu, 1, 2 ) = at( uu, 2, 1 );
    at( uu, 2, 0 ) = at( uu, 0, 2 );
-   const double s1 = at( T.dtr, 2, 1 ) - at( T.dtr, 1, 2 );
-   const double s2 = at( T.dtr, 0, 2 ) - at( T.dtr, 2, 0 );
-   const double s3 = at( T.dtr, 1, 0 ) - at( T.dtr, 0, 1 );
+   const R s1 = at( o.dtr, 2, 1 ) - at( o.dtr, 1, 2 );
+   const R s2 = at( o.dtr, 0, 2 ) - at( o.dtr, 2, 0 );
+   const R s3 = at( o.dtr, 1, 0 ) - at( o.dtr, 0, 1 );
    inverse3( uu, aux );
-   for ( int j = 0; j < 3; ++j ) T.udr[ j ] = ( at( aux, j, 0 ) * s1 + at( aux, j, 1 ) * s2 + at( aux, j, 2 ) * s3 );
-   cross_matrix( T.udr, uu );
+   for ( int j = 0; j < 3; ++j ) o.udr[ j ] = ( at( aux, j, 0 ) * s1 + at( aux, j, 1 ) * s2 + at( aux, j, 2 ) * s3 );
+   cross_matrix( o.udr, uu );
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j ) {
-         double t = 0.0;
+         R t = R( 0.0 );
          for ( int k = 0; k < 3; ++k )
             t = t + at( uu, i, k ) * at( drr, k, 0 ) * at( uu, 0, j ) + at( uu, i, k ) * at( drr, k, 1 ) * at( uu, 1, j ) +
                 at( uu, i, k ) * at( drr, k, 2 ) * at( uu, 2, j );
-         const double t1 = at( T.dtr, i, 0 ) * at( uu, 0, j ) + at( T.dtr, i, 1 ) * at( uu, 1, j ) + at( T.dtr, i, 2 ) * at( uu, 2, j );
-         const double t2 = -at( uu, i, 0 ) * at( T.dtr, j, 0 ) - at( uu, i, 1 ) * at( T.dtr, j, 1 ) - at( uu, i, 2 ) * at( T.dtr, j, 2 );
-         at( T.dtt, i, j ) = at( T.dtt, i, j ) - t + t1 + t2;
+         const R t1 = at( o.dtr, i, 0 ) * at( uu, 0, j ) + at( o.dtr, i, 1 ) * at( uu, 1, j ) + at( o.dtr, i, 2 ) * at( uu, 2, j );
+         const R t2 = -at( uu, i, 0 ) * at( o.dtr, j, 0 ) - at( uu, i, 1 ) * at( o.dtr, j, 1 ) - at( uu, i, 2 ) * at( o.dtr, j, 2 );
+         at( o.dtt, i, j ) = at( o.dtt, i, j ) - t + t1 + t2;
       }
    for ( int i = 0; i < 3; ++i )
       for ( int j = 0; j < 3; ++j )
-         at( T.dtr, i, j ) = at( T.dtr, i, j ) - at( uu, i, 0 ) * at( drr, 0, j ) - at( uu, i, 1 ) * at( drr, 1, j ) -
+         at( o.dtr, i, j ) = at( o.dtr, i, j ) - at( uu, i, 0 ) * at( drr, 0, j ) - at( uu, i, 1 ) * at( drr, 1, j ) -
                              at( uu, i, 2 ) * at( drr, 2, j );
+}
+
+BH_HD void difco_algebra( const double* s, double cg, Transport& T ) {
+   DifcoOut<double> o;
+   difco_t<double>( s, cg, o );
+   copy9( T.dtt, o.dtt ), copy9( T.dtr, o.dtr ), copy9( T.drr, o.drr );
+   for ( int k = 0; k < 3; ++k ) T.urc[ k ] = o.urc[ k ], T.udr[ k ] = o.udr[ k ];
+   copy9( T.ktt, o.ktt ), copy9( T.ktr, o.ktr ), copy9( T.krt, o.krt ), copy9( T.krr, o.krr );
 }
 
 BH_HD void visco_centre( const double* s, double d[ 3 ] ) {
@@ -436,13 +478,16 @@ BH_HD void visco_centre( const double* s, double d[ 3 ] ) {
    d[ 2 ] = at( bbi, 2, 0 ) * c1 + at( bbi, 2, 1 ) * c2 + at( bbi, 2, 2 ) * c3;
 }
 
-BH_HD double visco_chiprime( const double* s, double cg ) {
+template <typename R>
+BH_HD R chiprime_t( const R* s, double cg ) {
    const double pi = 3.1415926535897932385;
-   const double c112 = 0.5 * s[ 36 ], c213 = 0.5 * s[ 37 ], c323 = 0.5 * s[ 38 ];
-   double chip = 8 * pi * ( 0.2 * ( c112 + c213 + c323 ) + ( s[ 39 ] - 2 * s[ 40 ] + s[ 41 ] + s[ 42 ] + s[ 43 ] - 2 * s[ 44 ] ) / 15.0 );
+   const R c112 = 0.5 * s[ 36 ], c213 = 0.5 * s[ 37 ], c323 = 0.5 * s[ 38 ];
+   R chip = 8 * pi * ( 0.2 * ( c112 + c213 + c323 ) + ( s[ 39 ] - 2 * s[ 40 ] + s[ 41 ] + s[ 42 ] + s[ 43 ] - 2 * s[ 44 ] ) / 15.0 );
    if ( cg == 0 ) chip = 0.75 * chip;
    return chip;
 }
+
+BH_HD double visco_chiprime( const double* s, double cg ) { return chiprime_t<double>( s, cg ); }
 
 BH_HD void output_values( Transport& T ) {
    const double temperature = 293.15, eta = 0.01002, pi = 3.141592653;
@@ -455,6 +500,28 @@ BH_HD void output_values( Transport& T ) {
    copy9( a, T.dtt );
    eigen3( a, T.dtt_eig );
    T.dtrans = tunit * ( ( T.dtt_eig[ 0 ] + T.dtt_eig[ 1 ] + T.dtt_eig[ 2 ] ) / 3. );
+}
+
+// d(D_t, D_r, CHI * V)/d(sum q) for the first nslots sums, exactly (forward mode through the
+// algebra: one pass per sum).  D_t and D_r as output_values prints them are 1/3 of the traces of
+// Dtt and Drr in units (JACOBI's eigenvalues add up to the trace it reads), so their derivatives
+// are those of the traces; CHI * V is linear in the sums.  ghat[o * nslots + q].
+void algebra_gradient( const double* sums, double cg, int nslots, double* ghat ) {
+   const double temperature = 293.15, eta = 0.01002, pi = 3.141592653;
+   const double tunit = ( 1.38066E-16 ) * temperature * 1.0E8 / ( 8.0 * eta * pi );
+   const double runit = tunit * 1.0e16;
+   Dual s[ kSumSlots ];
+   for ( int k = 0; k < kSumSlots; ++k ) s[ k ] = Dual( sums[ k ] );
+   for ( int q = 0; q < nslots; ++q ) {
+      s[ q ].d = 1.0;
+      DifcoOut<Dual> o;
+      difco_t<Dual>( s, cg, o );
+      const Dual chip = chiprime_t<Dual>( s, cg );
+      ghat[ 0 * nslots + q ] = tunit * ( ( o.dtt[ 0 ].d + o.dtt[ 4 ].d + o.dtt[ 8 ].d ) / 3. );
+      ghat[ 1 * nslots + q ] = runit * ( ( o.drr[ 0 ].d + o.drr[ 4 ].d + o.drr[ 8 ].d ) / 3. );
+      ghat[ 2 * nslots + q ] = chip.d;
+      s[ q ].d = 0.0;
+   }
 }
 
 // JACOBI's eigenvalues of a symmetric 3x3 matrix (upper triangle of a, column-major; a is not

@@ -53,6 +53,16 @@ def example_two(oracle):
     return oracle.read_best(os.path.join(GOLDEN, "1aki49_atom_01806"))
 
 
+def no_torch_loaded():
+    """Tests that capture HIP graphs (the series) are not run once torch has been imported in the
+    process: with torch's own HIP libraries loaded beside the library's, an 8-mesh series capture
+    ended the process with a segmentation fault (host side, no GPU fault), and in another run a
+    larger 3-mesh capture did; the cause is not shown, so that state is kept out.  The one test
+    that imports torch (test_zz_autograd.py) runs after them."""
+    if "torch" in sys.modules:
+        pytest.skip("graph capture is not run after torch's HIP libraries are loaded in this process")
+
+
 def rel(a, b):
     return abs(a - b) / abs(b)
 

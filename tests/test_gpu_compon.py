@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 import pytest
-from conftest import GOLDEN
+from conftest import GOLDEN, no_torch_loaded
 
 pytestmark = pytest.mark.gpu
 
@@ -133,6 +133,7 @@ def test_signed_zero_and_nan(bh, oracle):
 
 
 def test_mesh_file_solves_to_the_same_bits(bh, tmp_path):
+    no_torch_loaded()
     from best_hydro_amd import meshes as M
     path = os.path.join(GOLDEN, "2eu7_02562")
     a = bh.Solver(0)

@@ -18,7 +18,7 @@ backward-stable solves differ by about cond(GR) u at this size).
 import numpy as np
 import pytest
 import transport_checks as TC
-from conftest import rel
+from conftest import rel, no_torch_loaded
 
 pytestmark = pytest.mark.gpu
 
@@ -113,6 +113,7 @@ def test_c2_against_oracle(bh, oracle, c2, prec):
 def series(bh):
     """The series exactly as the benchmark runs it: every mesh resident, autotuned, one launch of
     the captured graph (a warm launch first)."""
+    no_torch_loaded()
     from best_hydro_amd import meshes as M
     tris = M.series_meshes()
     print("series sizes", [t.nar for t in tris], flush=True)
@@ -193,6 +194,7 @@ def test_refinement_fallback_to_row_exchanges(bh, delaunay):
 def test_series_resolves_a_mesh_past_the_growth_threshold(bh, oracle, delaunay):
     """In the series, a mesh whose captured factorization met a multiplier above the threshold is
     solved again eagerly with row exchanges; the other mesh keeps its graph solve."""
+    no_torch_loaded()
     from best_hydro_amd import meshes as M
     t, ref = delaunay
     sphere = M.sphere_mesh(3, 15.0)
@@ -281,6 +283,7 @@ def test_device_algebra_matches_host(bh, prec):
 def test_series_graph_centre_of_viscosity(bh, oracle):
     """icov 1 in the captured graph (the centre computed and the second reduction run on the
     device, no host step between): against the oracle's run with the centre of viscosity."""
+    no_torch_loaded()
     import os
     from conftest import GOLDEN
     from best_hydro_amd import meshes as M

@@ -9,8 +9,8 @@ Euler's theorem).
 
 GPU: the adjoint (bhg_gradient) against that CPU reference gradient on a 320-triangle mesh, against
 central differences of the GPU solve itself on the 5,120-triangle bumpy ellipsoid and the
-two-surface mesh (per coordinate on a seeded subset of vertices and along random directions),
-against the same invariants at full size, and through torch.autograd.gradcheck.
+two-surface mesh (per coordinate on a seeded subset of vertices and along random directions), and
+against the same invariants at full size (torch.autograd.gradcheck: tests/test_zz_autograd.py).
 """
 import numpy as np
 import pytest
@@ -189,19 +189,3 @@ def test_gradient_scaling_identity(bh, mt_small):
         print(KEYS[o], t, r, e, flush=True)
         assert t < 1e-9 and r < 1e-9 and e < 1e-9
 
-
-@pytest.mark.gpu
-def test_autograd_gradcheck(bh, mt_small):
-    import torch
-    from best_hydro_amd.autograd import MeshTransport, transport
-    mt = MeshTransport(tiny_mesh(1), solver=mt_small.solver, precision=64, wm=14000.0)
-    V = torch.tensor(mt.V, dtype=torch.float64, requires_grad=True)
-    q0 = torch.tensor(mt.values(mt.V), dtype=torch.float64)
-    f = lambda X: transport(X, mt) / q0
-    assert torch.autograd.gradcheck(f, (V,), eps=1e-6, atol=1e-7, rtol=1e-5)
-    # backward of a scalar combination equals the combination of the rows
-    y = (transport(V, mt) * torch.tensor([1.0, 2.0, 3.0, 4.0], dtype=torch.float64) / q0).sum()
-    y.backward()
-    _, G = mt.values_and_gradient(mt.V)
-    want = np.einsum("o,ovc->vc", np.array([1.0, 2.0, 3.0, 4.0]) / q0.numpy(), G)
-    assert np.allclose(V.grad.numpy(), want, rtol=1e-12, atol=0)

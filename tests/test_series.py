@@ -3,7 +3,7 @@ captured HIP graph solves every mesh; each mesh's results equal its own single s
 oracle's; the 1/N fit equals numpy's."""
 import numpy as np
 import pytest
-from conftest import rel
+from conftest import rel, no_torch_loaded
 
 
 def test_extrapolation_matches_numpy(bh):
@@ -20,6 +20,7 @@ def test_extrapolation_matches_numpy(bh):
 
 @pytest.mark.gpu
 def test_series_graph_matches_single_solves_and_oracle(bh, oracle):
+    no_torch_loaded()
     from best_hydro_amd import meshes as M
     sizes = (2500, 3143, 3786)
     tris = M.series_meshes(sizes=sizes)
@@ -49,6 +50,7 @@ def test_series_graph_matches_single_solves_and_oracle(bh, oracle):
 @pytest.mark.gpu
 def test_series_fixed_tuning_and_icov(bh):
     """Hand-set tiles (no autotune) give the same answers; -vc needs the eager second pass."""
+    no_torch_loaded()
     from best_hydro_amd import meshes as M
     tris = M.series_meshes(sizes=(2500, 3143))
     ser = bh.Series(tris)
