@@ -33,6 +33,7 @@ METRIC = "mesh solves/s: FP32-MFMA LU+FP64 IR vs FP64 LU at 1e-10 rel err in D,[
 PEAKS = {  # MI355X published dense peaks
     "fp64_valu_tflops": 78.6, "fp32_mfma_tflops": 157.3, "fp64_mfma_tflops": 78.6, "hbm_gbs": 8000.0,
 }
+PAIR_FLOP = 3457.0  # FP64 flops of one triangle pair's 3x3 integral (special functions counted once)
 
 
 def _package():
@@ -129,6 +130,14 @@ def hmatrix_c5(bh, M, steps):
             "rank_mean": round(st["kmean"], 1), "rank_mean_before_recompression": round(st["kmean_aca"], 1),
             "recompression_s": round(st["t_trunc"], 3), "blocks_not_recompressed": st["n_trunc_failed"],
             "dtrans": r["dtrans"], "dr": r["dr"],
+            # the build against the FP64 VALU roofline of its kernel evaluations: every 3x3 block the
+            # cross approximations and the dense blocks evaluate, at the pair integral's 3,457 flop
+            # (special functions counted once), over the build's fill time (recompression inside)
+            "build_roofline": {"kernel_blocks": st["kernel_blocks"], "flop": st["kernel_blocks"] * PAIR_FLOP,
+                               "fill_s": round(st["t_fill"], 3),
+                               "achieved": round(st["kernel_blocks"] * PAIR_FLOP / st["t_fill"] / 1e12, 2),
+                               "peak": PEAKS["fp64_valu_tflops"], "unit": "TFLOP/s",
+                               "frac": round(st["kernel_blocks"] * PAIR_FLOP / st["t_fill"] / 1e12 / PEAKS["fp64_valu_tflops"], 4)},
             "roofline": {"kernel": "H-matrix product (k_hm_vtx + k_hm_rows + sums)", "bound": "hbm",
                          "achieved": round(gbs, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
                          "frac": round(gbs / PEAKS["hbm_gbs"], 4), "avg_product_ms": round(ms, 4),
@@ -374,7 +383,7 @@ def main():
     # triangular solves and refinement residuals as the bytes of the factors and of GR they stream.
     npad = st32["npad"]
     lu_flops = 2.0 / 3.0 * npad ** 3
-    pair_flops = 3457.0 * n * (n - 1.0)
+    pair_flops = PAIR_FLOP * n * (n - 1.0)
     super_blk = 1024
     f32_solve_bytes = 4.0 * (npad * npad + 2 * npad * super_blk)   # L, U and the super-block inverses
     f64_solve_bytes = 8.0 * (npad * npad + 2 * npad * super_blk)
