@@ -354,7 +354,7 @@ def main():
 
     # Dominant kernel class of the mixed path, by measured time.
     flop_classes = {"pairs": ("fp64_valu_tflops", "TFLOP/s"), "gemm32": ("fp32_mfma_tflops", "TFLOP/s"),
-                    "gemm32_128": ("fp32_mfma_tflops", "TFLOP/s"),
+                    "gemm32_128": ("fp32_mfma_tflops", "TFLOP/s"), "gemm32_64": ("fp32_mfma_tflops", "TFLOP/s"),
                     "gemm64": ("fp64_mfma_tflops", "TFLOP/s"), "diag": ("fp32_mfma_tflops", "TFLOP/s"),
                     "trsm": ("fp32_mfma_tflops", "TFLOP/s"), "resid": ("fp64_valu_tflops", "TFLOP/s")}
     dom = max((k for k in prof32 if prof32[k]["launches"] > 0 and k in flop_classes), key=lambda k: prof32[k]["ms"])
@@ -365,7 +365,8 @@ def main():
     peak = PEAKS[flop_classes[dom][0]]
     traffic, traffic_src = traffic_from_profile(dom)
     kernels = {"gemm32_128": "k_gemm<float, 16, 128, 128> (FP32 trailing updates, 128 x 128 tiles)",
-               "gemm32": "k_gemm<float, 32, *, *> (FP32 panels and narrow updates, 64-wide tiles)",
+               "gemm32": "k_gemm<float, 32, 64, 128> and <float, 32, 128, 64> (FP32 panels and U row panels)",
+               "gemm32_64": "k_gemm<float, 32, 64, 64> (FP32 narrow update parts, 64 x 64 tiles)",
                "pairs": "k_pairs<true> (assembly)", "diag": "k_lu_diag<float>"}
     roofline = {"bound": "mfma" if dom.startswith("gemm") else "valu", "kernel_class": dom,
                 "kernel": kernels.get(dom, dom),

@@ -308,8 +308,9 @@ class Solver:
         self.L.bhg_get_stats(self.h, C.byref(s))
         return s.as_dict()
 
-    # gemm32: FP32 GEMM launches on tiles below 128 x 128; gemm32_128: on 128 x 128 tiles
-    PROFILE_CLASSES = ("pairs", "gemm64", "gemm32", "diag", "trsm", "resid", "other", "gemm32_128")
+    # FP32 GEMM launches by tile: gemm32 on 64 x 128 and 128 x 64 (panels, U row panels),
+    # gemm32_64 on 64 x 64 (narrow update parts), gemm32_128 on 128 x 128 (wide trailing updates)
+    PROFILE_CLASSES = ("pairs", "gemm64", "gemm32", "diag", "trsm", "resid", "other", "gemm32_128", "gemm32_64")
 
     def set_profiling(self, on=True):
         self.L.bhg_set_profiling(self.h, int(bool(on)))

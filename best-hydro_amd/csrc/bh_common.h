@@ -52,9 +52,11 @@ struct LuStreams {
 };
 
 // Kernel classes timed by the optional launch profiler.
-// kKGemm32 counts the FP32 GEMMs on tiles smaller than 128 x 128 (the panels, row panels and the
-// narrow parts of the trailing update), kKGemm32T those on 128 x 128 tiles (the wide trailing updates).
-enum KernelClass { kKPairs = 0, kKGemm64, kKGemm32, kKDiag, kKTrsm, kKResid, kKOther, kKGemm32T, kKClasses };
+// kKGemm32 counts the FP32 GEMMs on 64 x 128 and 128 x 64 tiles (the panels below the diagonal
+// blocks and the U row panels, products written in place), kKGemm32S those on 64 x 64 tiles (the
+// narrow parts of the trailing update and the column updates between the blocks of a pair),
+// kKGemm32T those on 128 x 128 tiles (the wide trailing updates).
+enum KernelClass { kKPairs = 0, kKGemm64, kKGemm32, kKDiag, kKTrsm, kKResid, kKOther, kKGemm32T, kKGemm32S, kKClasses };
 
 // When a solver turns profiling on, every launch of a timed class is bracketed by two HIP events
 // on the launching stream; the algorithmic flops of the launch are added to its class.  The

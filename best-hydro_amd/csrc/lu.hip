@@ -69,41 +69,48 @@ struct GemmOcc {
    static constexpr int waves = ( sizeof( T ) == 4 && BK == 16 ) ? 3 : sizeof( T ) == 8 ? 2 : 1;
 };
 
-template <typename T, int BK, int BM = 128, int BN = 128>
-__global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
-                                                T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
-                                                BatchStride sb, BatchStride sc ) {
-   constexpr int PAD = 8;
+// The operands of one GEMM problem (as k_gemm's arguments).
+template <typename T>
+struct GemmProb {
+   int M, N, K;
+   T alpha;
+   const T* A;
+   long lda;
+   const T* B;
+   long ldb;
+   T beta;
+   T* C;
+   long ldc;
+   int* check;
+   T thr;
+};
+
+constexpr int kGemmPad = 8;
+// LDS elements of one workgroup's two-stage A and B slices.
+template <int BK, int BM, int BN>
+constexpr int gemm_lds_elems() {
+   return 2 * BK * ( BM + kGemmPad ) + 2 * BK * ( BN + kGemmPad );
+}
+
+// Tile (bm, bn) of problem g, by the 256 threads of the workgroup; smem holds gemm_lds_elems().
+template <typename T, int BK, int BM, int BN>
+__device__ inline void gemm_tile( int bm, int bn, const GemmProb<T>& g, T* smem ) {
+   constexpr int PAD = kGemmPad;
    constexpr int WM = BM / 2, WN = BN / 2;  // each wave's block of the tile
    constexpr int TM = WM / 16, TN = WN / 16;  // its MFMA 16 x 16 blocks
-   __shared__ __attribute__( ( aligned( 16 ) ) ) T As[ 2 ][ BK ][ BM + PAD ];
-   __shared__ __attribute__( ( aligned( 16 ) ) ) T Bs[ 2 ][ BK ][ BN + PAD ];
+   T* As = smem;                            // [2][BK][BM + PAD]
+   T* Bs = smem + 2 * BK * ( BM + PAD );    // [2][BK][BN + PAD]
+   auto as = [&]( int stg, int k, int m ) -> T& { return As[ ( stg * BK + k ) * ( BM + PAD ) + m ]; };
+   auto bs = [&]( int stg, int k, int n ) -> T& { return Bs[ ( stg * BK + k ) * ( BN + PAD ) + n ]; };
    using v4 = typename Mfma<T>::v4;
-   {
-      const long q = blockIdx.z / zdiv, t = blockIdx.z % zdiv;
-      A += q * sa.hi + t * sa.lo;
-      B += q * sb.hi + t * sb.lo;
-      C += q * sc.hi + t * sc.lo;
-   }
+   const int K = g.K;
+   const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+   const T alpha = g.alpha, beta = g.beta;
+   T* C = g.C;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave & 1, wn = wave >> 1;
-   // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs (workgroup id % 8), each
-   // XCD with its own L2: give each XCD a contiguous run of the tiles, and order the tiles in
-   // groups of kGroupM block rows, column by column, so that the tiles running together on one
-   // XCD share a few row strips of A and column strips of B in its L2.
-   int bm, bn;
-   {
-      constexpr int kXcd = 8, kGroupM = 8;
-      const int nx = gridDim.x, ntile = nx * gridDim.y;
-      const int id0 = blockIdx.y * nx + blockIdx.x;
-      const int xcd = id0 % kXcd, loc = id0 / kXcd, base = ntile / kXcd, rem = ntile % kXcd;
-      const int id = ( xcd < rem ? xcd * ( base + 1 ) : rem * ( base + 1 ) + ( xcd - rem ) * base ) + loc;
-      const int per = kGroupM * gridDim.y, first = ( id / per ) * kGroupM, gm = min( nx - first, kGroupM );
-      bm = first + ( id % per ) % gm;
-      bn = ( id % per ) / gm;
-   }
-   const T* Ab = A + (long)bm * BM;
-   const T* Bb = B + (long)bn * BN * ldb;
+   const T* Ab = g.A + (long)bm * BM;
+   const T* Bb = g.B + (long)bn * BN * ldb;
    constexpr int AE = BM * BK / 256;  // A elements per thread, contiguous along m
    constexpr int ATPK = BM / AE;      // threads per k column of A
    constexpr int BE = BK * BN / 256;  // B elements per thread, contiguous along k
@@ -139,9 +146,9 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( G
    };
    auto sstore = [&]( int stg ) {
 #pragma unroll
-      for ( int e = 0; e < AE; ++e ) As[ stg ][ a_k ][ slot( a_m + e, WM ) ] = ra[ e ];
+      for ( int e = 0; e < AE; ++e ) as( stg, a_k, slot( a_m + e, WM ) ) = ra[ e ];
 #pragma unroll
-      for ( int e = 0; e < BE; ++e ) Bs[ stg ][ b_k + e ][ b_slot ] = rb[ e ];
+      for ( int e = 0; e < BE; ++e ) bs( stg, b_k + e, b_slot ) = rb[ e ];
    };
    gload( 0 );
    // the C tile after the first slice, so that storing the slice to LDS need not wait for it
@@ -163,8 +170,8 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( G
 #pragma unroll
       for ( int kk = 0; kk < BK; kk += 4 ) {
          const int kr = kk + lk;
-         const T* pa = &As[ stg ][ kr ][ wm * WM + li * TM ];
-         const T* pb = &Bs[ stg ][ kr ][ wn * WN + li * TN ];
+         const T* pa = &as( stg, kr, wm * WM + li * TM );
+         const T* pb = &bs( stg, kr, wn * WN + li * TN );
          T af[ TM ], bf[ TN ];
 #pragma unroll
          for ( int t = 0; t < TM; ++t ) af[ t ] = pa[ t ];
@@ -189,10 +196,53 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( G
             T v = alpha * acc[ t ][ u ][ r ];
             if ( beta != T( 0 ) ) v = v + beta * ( kPre ? cpre[ t ][ u ][ r ] : c[ crow( t, r ) ] );
             c[ crow( t, r ) ] = v;
-            big |= !( tabs( v ) <= thr );
+            big |= !( tabs( v ) <= g.thr );
          }
       }
-   if ( check && big ) atomicOr( check, 1 );
+   if ( g.check && big ) atomicOr( g.check, 1 );
+}
+
+template <typename T, int BK, int BM = 128, int BN = 128>
+__global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
+                                                T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
+                                                BatchStride sb, BatchStride sc ) {
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T smem[ gemm_lds_elems<BK, BM, BN>() ];
+   {
+      const long q = blockIdx.z / zdiv, t = blockIdx.z % zdiv;
+      A += q * sa.hi + t * sa.lo;
+      B += q * sb.hi + t * sb.lo;
+      C += q * sc.hi + t * sc.lo;
+   }
+   // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs (workgroup id % 8), each
+   // XCD with its own L2: give each XCD a contiguous run of the tiles, and order the tiles in
+   // groups of kGroupM block rows, column by column, so that the tiles running together on one
+   // XCD share a few row strips of A and column strips of B in its L2.
+   int bm, bn;
+   {
+      constexpr int kXcd = 8, kGroupM = 8;
+      const int nx = gridDim.x, ntile = nx * gridDim.y;
+      const int id0 = blockIdx.y * nx + blockIdx.x;
+      const int xcd = id0 % kXcd, loc = id0 / kXcd, base = ntile / kXcd, rem = ntile % kXcd;
+      const int id = ( xcd < rem ? xcd * ( base + 1 ) : rem * ( base + 1 ) + ( xcd - rem ) * base ) + loc;
+      const int per = kGroupM * gridDim.y, first = ( id / per ) * kGroupM, gm = min( nx - first, kGroupM );
+      bm = first + ( id % per ) % gm;
+      bn = ( id % per ) / gm;
+   }
+   gemm_tile<T, BK, BM, BN>( bm, bn, GemmProb<T> { M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, thr }, smem );
+}
+
+// Two independent products in one launch, for the factorization's critical path: p0 on 64 x 128
+// tiles (the panel below a diagonal block, written over A: N == 128) and p1 on 128 x 64 tiles (a U
+// row panel, written over B: M == 128).  Workgroups 0 .. t0 - 1 take p0's tiles, the rest p1's.
+template <typename T, int BK>
+__global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm_pair( GemmProb<T> p0, GemmProb<T> p1 ) {
+   static_assert( gemm_lds_elems<BK, 64, 128>() == gemm_lds_elems<BK, 128, 64>(), "one LDS size for both shapes" );
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T smem[ gemm_lds_elems<BK, 64, 128>() ];
+   const int t0 = p0.M / 64;
+   if ( (int)blockIdx.x < t0 )
+      gemm_tile<T, BK, 64, 128>( blockIdx.x, 0, p0, smem );
+   else
+      gemm_tile<T, BK, 128, 64>( 0, blockIdx.x - t0, p1, smem );
 }
 
 // ------------------------------------------------------------------ the diagonal block
@@ -927,16 +977,24 @@ __global__ void __launch_bounds__( 256 ) k_resid_part( int npad, int chunk, int 
 #pragma unroll
    for ( int m = 0; m < 4; ++m ) ok[ m ] = row0 + 256 * m < npad;
    const double* a = A + (long)c0 * lda + row0;
-   for ( int j = 0; j < chunk; ++j ) {
-      double v[ 4 ];
+   // four columns' loads issued before their products (16 loads in flight per lane); the
+   // accumulation order over j is unchanged
+   constexpr int U = 4;
+   for ( int j = 0; j < chunk; j += U ) {
+      double v[ U ][ 4 ];
 #pragma unroll
-      for ( int m = 0; m < 4; ++m ) v[ m ] = ok[ m ] ? a[ (long)j * lda + 256 * m ] : 0.0;
-      const double* xr = xs + j * 16;
+      for ( int u = 0; u < U; ++u )
 #pragma unroll
-      for ( int r = 0; r < 16; ++r ) {
-         const double xv = xr[ r ];
+         for ( int m = 0; m < 4; ++m ) v[ u ][ m ] = ok[ m ] ? a[ (long)( j + u ) * lda + 256 * m ] : 0.0;
 #pragma unroll
-         for ( int m = 0; m < 4; ++m ) acc[ m ][ r ] = acc[ m ][ r ] + v[ m ] * xv;
+      for ( int u = 0; u < U; ++u ) {
+         const double* xr = xs + ( j + u ) * 16;
+#pragma unroll
+         for ( int r = 0; r < 16; ++r ) {
+            const double xv = xr[ r ];
+#pragma unroll
+            for ( int m = 0; m < 4; ++m ) acc[ m ][ r ] = acc[ m ][ r ] + v[ u ][ m ] * xv;
+         }
       }
    }
    double* out = part + (long)blockIdx.y * 16 * npad;
@@ -1043,7 +1101,8 @@ void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T*
    const long tiles = (long)( M / 128 ) * ( N / 128 ) * batch;
    constexpr long kSmall = 256;  // CUs
    const int shape = tiles >= kSmall ? 0 : C == A && N == 128 ? 1 : C == B && M == 128 ? 2 : C != A && C != B ? 3 : 0;
-   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : shape == 0 ? kKGemm32T : kKGemm32, 2.0 * M * N * K * batch, st );
+   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : shape == 0 ? kKGemm32T : shape == 3 ? kKGemm32S : kKGemm32,
+                   2.0 * M * N * K * batch, st );
    if ( shape == 1 )
       launch_gemm_shape<T, 64, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
    else if ( shape == 2 )
@@ -1052,6 +1111,23 @@ void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T*
       launch_gemm_shape<T, 64, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
    else
       launch_gemm_shape<T, 128, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+}
+
+// k_gemm_pair with the K slice launch_gemm_shape would choose for these tile shapes.
+template <typename T>
+void launch_gemm_pair( hipStream_t st, const GemmProb<T>& p0, const GemmProb<T>& p1 ) {
+   if ( p0.N != 128 || p1.M != 128 || p0.M % 64 != 0 || p1.N % 64 != 0 || p0.K != p1.K || p0.K % 64 != 0 )
+      throw std::runtime_error( "k_gemm_pair: shapes outside its tiles" );
+   const dim3 grid( p0.M / 64 + p1.N / 64 );
+   if ( grid.x == 0 ) return;
+   if constexpr ( sizeof( T ) == 8 )
+      hipLaunchKernelGGL( ( k_gemm_pair<double, 16> ), grid, dim3( 256 ), 0, st, p0, p1 );
+   else if ( g_gemm_bk_f32 == 16 )
+      hipLaunchKernelGGL( ( k_gemm_pair<float, 16> ), grid, dim3( 256 ), 0, st, p0, p1 );
+   else if ( g_gemm_bk_f32 == 64 )
+      hipLaunchKernelGGL( ( k_gemm_pair<float, 64> ), grid, dim3( 256 ), 0, st, p0, p1 );
+   else
+      hipLaunchKernelGGL( ( k_gemm_pair<float, 32> ), grid, dim3( 256 ), 0, st, p0, p1 );
 }
 
 template <typename T>
@@ -1079,6 +1155,10 @@ template void launch_gemm<float>( hipStream_t, int, int, int, float, const float
 // beside the second block column's factorization, so st runs only diagonal block, panel, update,
 // diagonal block, panel, next-columns update.  check is set when partial pivoting would have exchanged
 // rows somewhere; the caller must then rerun lu_pivoted on the original matrix.
+// Trailing sizes (rows after the pair) from which part B of the update is deferred every other
+// pair (rank-512 updates).
+constexpr int kDeferMinRows = 6144;
+
 template <typename T>
 void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check ) {
    const int nb = kNB, nblk = npad / nb;
@@ -1090,12 +1170,19 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
       ProfScope prof( kKDiag, 4.0 / 3.0 * nb * nb * nb, st );  // LU 2/3 nb^3 + two triangular inverses nb^3 / 3 each
       hipLaunchKernelGGL( k_lu_diag<T>, dim3( 1 ), dim3( 256 ), 0, st, 0, blk( k, k ), lda, inv_l( k ), check, T( g_growth ) );
    };
-   auto lpanel = [&]( int k ) {  // the panel below the diagonal block: A U^-1 (checks |l| <= 1)
-      launch_gemm<T>( st, npad - ( k + 1 ) * nb, nb, nb, T( 1 ), blk( k + 1, k ), lda, inv_u( k ), nb, T( 0 ),
-                      blk( k + 1, k ), lda, check );
-   };
    auto urow = [&]( hipStream_t s, int k, int c0, int ncols ) {  // U row panel: L^-1 A over block columns c0..
       launch_gemm<T>( s, nb, ncols, nb, T( 1 ), inv_l( k ), nb, blk( k, c0 ), lda, T( 0 ), blk( k, c0 ), lda, nullptr );
+   };
+   // The panel below diagonal block k and the U row panel U(k, c0 .. c0 + ncols) in one launch
+   // (both read only block k's inverses and what is already updated).
+   auto panel_urow = [&]( int k, int c0, int ncols ) {
+      const int m0 = npad - ( k + 1 ) * nb;
+      const GemmProb<T> p0 { m0, nb, nb, T( 1 ), blk( k + 1, k ), lda, inv_u( k ), nb, T( 0 ), blk( k + 1, k ), lda, check,
+                             T( g_growth ) };
+      const GemmProb<T> p1 { nb, ncols, nb, T( 1 ), inv_l( k ), nb, blk( k, c0 ), lda, T( 0 ), blk( k, c0 ), lda, nullptr,
+                             T( 0 ) };
+      ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32, 2.0 * ( (double)m0 + ncols ) * nb * nb, st );
+      launch_gemm_pair<T>( st, p0, p1 );
    };
    auto order = [&]( hipEvent_t e, hipStream_t from, hipStream_t to ) {  // to waits for what from has so far
       BH_HIP( hipEventRecord( e, from ) );
@@ -1104,6 +1191,7 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
    bool pending = false;    // an update on st2 (part A) that the next pair's U row panels must wait for
    bool pending_n = false;  // an update on st2 (part N) that the next pair's next-columns update must wait for
    bool used2 = false;      // st2 has work to join at the end
+   bool deferred = false;   // the previous pair's part B is still to be applied (with this pair's)
    for ( int k0 = 0; k0 < nblk; k0 += 2 ) {
       const int k1 = k0 + 1;
       if ( k1 >= nblk ) {  // a last single block (npad an odd multiple of 128)
@@ -1112,36 +1200,39 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
       }
       const bool last = k1 + 1 == nblk;
       const int c2 = k1 + 1, rest = npad - c2 * nb;
+      const int nnext = std::min( 2 * nb, rest );
       diag( k0 );
-      order( S.ev[ 0 ], st, st3 );
-      urow( st3, k0, k1, nb );
-      BH_HIP( hipEventRecord( S.ev[ 1 ], st3 ) );
-      if ( !last ) {
+      if ( !last ) {  // U(k0, c2..) on st3, once part A of the previous update has reached those rows
+         order( S.ev[ 0 ], st, st3 );
          if ( pending ) BH_HIP( hipStreamWaitEvent( st3, S.ev_g, 0 ) );
          pending = false;
          urow( st3, k0, c2, rest );
       }
-      lpanel( k0 );
+      panel_urow( k0, k1, nb );  // the panel below k0 and U(k0, k1)
       if ( !last ) {
          // rows of block k1 right of the pair: A -= L(k1, k0) U(k0, c2..), on st3 once the panel is done
          order( S.ev[ 2 ], st, st3 );
          launch_gemm<T>( st3, nb, rest, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, c2 ), lda, T( 1 ), blk( k1, c2 ), lda,
                          nullptr );
+         BH_HIP( hipEventRecord( S.ev[ 1 ], st3 ) );
       }
-      BH_HIP( hipStreamWaitEvent( st, S.ev[ 1 ], 0 ) );
       launch_gemm<T>( st, npad - k1 * nb, nb, nb, T( -1 ), blk( k1, k0 ), lda, blk( k0, k1 ), lda, T( 1 ), blk( k1, k1 ), lda,
                       nullptr );
       diag( k1 );
       if ( last ) break;
-      order( S.ev[ 3 ], st, st3 );
-      urow( st3, k1, c2, rest );
-      BH_HIP( hipEventRecord( S.ev[ 4 ], st3 ) );
-      lpanel( k1 );
-      BH_HIP( hipStreamWaitEvent( st, S.ev[ 4 ], 0 ) );
+      // the panel below k1 and U(k1, next pair), once block row k1 has its update by k0 (st3)
+      BH_HIP( hipStreamWaitEvent( st, S.ev[ 1 ], 0 ) );
+      panel_urow( k1, c2, nnext );
       // the rank-256 trailing update: L = the two panels below, U = the two row panels
-      const int nnext = std::min( 2 * nb, rest );
       const bool more = rest > nnext;
-      if ( more ) order( S.ev_u, st, st2 );  // st2 reads this pair's panels, not the next columns
+      if ( more ) {
+         // st2 reads this pair's panels and U row panels, not the next columns.  It forms U(k1, ..)
+         // beyond the next pair itself first: st2 never waits for st3, which waits for st2 (part
+         // A).  Two forked streams waiting for each other break the graph capture of this schedule
+         // (the host process crashed in the capture when st2 waited for st3 here).
+         order( S.ev_u, st, st2 );
+         urow( st2, k1, c2 + nnext / nb, rest - nnext );
+      }
       // the next pair's columns, after the previous update's part N has reached them
       if ( pending_n ) BH_HIP( hipStreamWaitEvent( st, S.ev[ 5 ], 0 ) );
       pending_n = false;
@@ -1151,7 +1242,15 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
          // The rest on st2, in three parts so that the next pair waits only for what it reads:
          // A, the next pair's rows (its U row panels, on st3); N, the columns of the pair after it
          // (its next-columns update, on st); B, the remainder, which overlaps the next pair.
+         // While the trailing matrix is large, part B of every other pair is deferred and applied
+         // with the next pair's: one rank-512 product, which runs the GEMM closer to its peak than
+         // two rank-256 ones.  The deferring pair applies at once, besides A and N, what the next
+         // pair's own parts A and N will cover (A2: the rows of the pair after next, right of it;
+         // N2: the columns of the pair after that), so that the next pair's A and N stay rank-256
+         // and nothing the next two pairs read waits for the deferred product.
          const int cr = c2 + nnext / nb, wr = rest - nnext, nn = std::min( 2 * nb, wr );
+         const int cr2 = cr + nn / nb, nn2 = std::min( 2 * nb, wr - nn );
+         const bool defer = !deferred && wr - nn - nn2 > 0 && rest >= kDeferMinRows;
          launch_gemm<T>( st2, nnext, wr, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( c2, cr ), lda,
                          nullptr );
          BH_HIP( hipEventRecord( S.ev_g, st2 ) );
@@ -1160,11 +1259,19 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
                          nullptr );
          BH_HIP( hipEventRecord( S.ev[ 5 ], st2 ) );
          pending_n = true;
-         if ( wr > nn )
-            launch_gemm<T>( st2, wr, wr - nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr + nn / nb ), lda, T( 1 ),
-                            blk( cr, cr + nn / nb ), lda, nullptr );
+         if ( defer ) {
+            launch_gemm<T>( st2, nn, wr - nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr2 ), lda, T( 1 ), blk( cr, cr2 ),
+                            lda, nullptr );
+            launch_gemm<T>( st2, wr - nn, nn2, 2 * nb, T( -1 ), blk( cr2, k0 ), lda, blk( k0, cr2 ), lda, T( 1 ),
+                            blk( cr2, cr2 ), lda, nullptr );
+         } else if ( wr > nn ) {
+            const int kf = deferred ? k0 - 2 : k0;  // first block column whose panels this product applies
+            launch_gemm<T>( st2, wr, wr - nn, ( c2 - kf ) * nb, T( -1 ), blk( cr, kf ), lda, blk( kf, cr2 ), lda, T( 1 ),
+                            blk( cr, cr2 ), lda, nullptr );
+         }
          BH_HIP( hipEventRecord( S.ev[ 6 ], st2 ) );
          used2 = true;
+         deferred = defer;
       }
    }
    // the caller's later work on st must see all of st2's updates (st3's work is joined into st

@@ -148,9 +148,10 @@ int bhg_set_option( bhg_solver* s, const char* name, double value );
 
 /* Launch profiling of bhg_solve_mesh: with on = 1 every launch of the timed kernel classes is
  * bracketed by HIP events on the solver's stream.  bhg_get_profile returns, per class (0 pair
- * assembly, 1 FP64 GEMM, 2 FP32 GEMM on tiles below 128 x 128, 3 diagonal blocks, 4 triangular
- * solves, 5 refinement residuals, 6 other, 7 FP32 GEMM on 128 x 128 tiles: 8 entries), the summed
- * duration (ms), algorithmic flops and launch count since profiling was turned on. */
+ * assembly, 1 FP64 GEMM, 2 FP32 GEMM on 64 x 128 and 128 x 64 tiles, 3 diagonal blocks,
+ * 4 triangular solves, 5 refinement residuals, 6 other, 7 FP32 GEMM on 128 x 128 tiles, 8 FP32
+ * GEMM on 64 x 64 tiles: 9 entries), the summed duration (ms), algorithmic flops and launch count
+ * since profiling was turned on. */
 int bhg_set_profiling( bhg_solver* s, int on );
 int bhg_get_profile( bhg_solver* s, double* ms, double* flops, long* count );
 
