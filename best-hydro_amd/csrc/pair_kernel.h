@@ -29,6 +29,41 @@ __constant__ double c_weight[ 21 ] = {
 
 __device__ inline double dot3( const double* p, const double* q ) { return p[ 0 ] * q[ 0 ] + p[ 1 ] * q[ 1 ] + p[ 2 ] * q[ 2 ]; }
 
+// a / b and sqrt(x) with the instructions the compiler emits for the IEEE operations, less their
+// range handling.  The quotient: the reciprocal (v_rcp_f64) refined by two Newton steps, the
+// product with a and one correction of it; the compiler's sequence runs the same steps on
+// operands that v_div_scale_f64 first scales when an exponent is near the ends of the range (and
+// v_div_fixup_f64 then handles zeros, infinities and NaNs).  The root: v_rsq_f64 refined by the
+// same coupled iteration for sqrt(x) and 1 / (2 sqrt(x)) the compiler runs after scaling x below
+// 2^-767 up by 2^256 (and before passing 0 and +inf through).  Every operand here is a length, a
+// squared distance or a product of a few, far from both ends of the double range and not zero
+// (the collocation point is strictly inside its own triangle, off the source triangle's edges),
+// so the scalings are identities and the results are the IEEE operations' bits, with a third
+// fewer instructions per quotient and half per root.
+__device__ __forceinline__ double qdiv( double a, double b ) {
+   double r = __builtin_amdgcn_rcp( b );
+   double e = fma( -b, r, 1.0 );
+   r = fma( r, e, r );
+   e = fma( -b, r, 1.0 );
+   r = fma( r, e, r );
+   const double q = a * r;
+   const double rem = fma( -b, q, a );
+   return fma( rem, r, q );
+}
+
+__device__ __forceinline__ double qsqrt( double x ) {
+   const double y = __builtin_amdgcn_rsq( x );
+   double g = x * y;
+   double h = y * 0.5;
+   const double r = fma( -h, g, 0.5 );
+   g = fma( g, r, g );
+   const double d0 = fma( -g, g, x );
+   h = fma( h, r, h );
+   g = fma( d0, h, g );
+   const double d1 = fma( -g, g, x );
+   return fma( d1, h, g );
+}
+
 // The off-diagonal block: source triangle record s (uniform across the wave), collocation point
 // c (this lane's incenter).  Writes the 6 distinct entries g11 g21 g22 g31 g32 g33.
 // best/triangleint.f:17-188 with the integrands h, hx, hy, hxy of :198-247; sqrt(d2121) and the
@@ -57,29 +92,29 @@ __device__ inline void pair_block( const double* __restrict__ s, const double c[
    for ( int j = 0; j < 21; ++j ) {
       const double y = c_node[ j ];
       const double w = c_weight[ j ];
-      const double q1 = sqrt( d1c1c + 2.0 * y * d1c31 + y * y * d3131 );
-      const double q2 = sqrt( d2c2c + 2.0 * y * d322c + y * y * d3232 );
+      const double q1 = qsqrt( d1c1c + 2.0 * y * d1c31 + y * y * d3131 );
+      const double q2 = qsqrt( d2c2c + 2.0 * y * d322c + y * y * d3232 );
       // h(y)
-      const double fh =
-          log( ( d212c + d2132 * y + s21 * q2 ) / ( d211c + d2131 * y + s21 * sqrt( d1c1c + 2.0 * d1c31 * y + d3131 * y * y ) ) ) /
-          s21;
+      const double fh = qdiv( log( qdiv( d212c + d2132 * y + s21 * q2,
+                                         d211c + d2131 * y + s21 * qsqrt( d1c1c + 2.0 * d1c31 * y + d3131 * y * y ) ) ),
+                              s21 );
       // hx(y)
-      double a = ( -d1c31 - y * d2131 ) / sqrt( d1c1c + 2.0 * y * d211c + y * y * d2121 );
-      double b = ( -y * d3231 + d3c31 ) / sqrt( y * y * d3232 - 2.0 * y * d3c32 + d3c3c );
+      double a = qdiv( -d1c31 - y * d2131, qsqrt( d1c1c + 2.0 * y * d211c + y * y * d2121 ) );
+      double b = qdiv( -y * d3231 + d3c31, qsqrt( y * y * d3232 - 2.0 * y * d3c32 + d3c3c ) );
       double q = ( -d1c31 * d1c31 - 2.0 * y * d1c31 * d2131 - y * y * d2131 * d2131 + d1c1c * d3131 +
                    2.0 * y * d211c * d3131 + y * y * d2121 * d3131 );
-      const double fhx = ( a + b ) / q;
+      const double fhx = qdiv( a + b, q );
       // hy(y)
-      a = ( d211c + y * d2131 ) / q1;
-      b = ( y * d2132 + d212c ) / q2;
+      a = qdiv( d211c + y * d2131, q1 );
+      b = qdiv( y * d2132 + d212c, q2 );
       q = ( d211c * d211c - d1c1c * d2121 - 2.0 * y * d1c31 * d2121 + 2.0 * y * d211c * d2131 + y * y * d2131 * d2131 -
             y * y * d2121 * d3131 );
-      const double fhy = ( a - b ) / q;
+      const double fhy = qdiv( a - b, q );
       // hxy(y)
-      b = ( y * d211c - d2c1c - 2.0 * y * d1c31 - y * d2131 - y * y * d3231 ) / q2;
+      b = qdiv( y * d211c - d2c1c - 2.0 * y * d1c31 - y * d2131 - y * y * d3231, q2 );
       q = ( -d211c * d211c + d1c1c * d2121 + 2.0 * y * d1c31 * d2121 - 2.0 * y * d211c * d2131 - y * y * ( d2131 * d2131 ) +
             y * y * d2121 * d3131 );
-      const double fhxy = ( q1 + b ) / q;
+      const double fhxy = qdiv( q1 + b, q );
       t0 = t0 + w * fh;
       s0 = s0 + w * fhy;
       sx = sx + w * fhxy;
