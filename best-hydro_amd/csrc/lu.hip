@@ -207,6 +207,9 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( G
                                                 T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
                                                 BatchStride sb, BatchStride sc ) {
    __shared__ __attribute__( ( aligned( 16 ) ) ) T smem[ gemm_lds_elems<BK, BM, BN>() ];
+   // the smaller tiles carry the factorization's critical path: their waves take issue
+   // priority over the wide trailing updates' waves sharing a CU
+   if constexpr ( BM < 128 || BN < 128 ) __builtin_amdgcn_s_setprio( 2 );
    {
       const long q = blockIdx.z / zdiv, t = blockIdx.z % zdiv;
       A += q * sa.hi + t * sa.lo;
@@ -238,6 +241,7 @@ template <typename T, int BK>
 __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm_pair( GemmProb<T> p0, GemmProb<T> p1 ) {
    static_assert( gemm_lds_elems<BK, 64, 128>() == gemm_lds_elems<BK, 128, 64>(), "one LDS size for both shapes" );
    __shared__ __attribute__( ( aligned( 16 ) ) ) T smem[ gemm_lds_elems<BK, 64, 128>() ];
+   __builtin_amdgcn_s_setprio( 2 );  // the critical path (as the smaller tiles of k_gemm)
    const int t0 = p0.M / 64;
    if ( (int)blockIdx.x < t0 )
       gemm_tile<T, BK, 64, 128>( blockIdx.x, 0, p0, smem );
@@ -520,6 +524,7 @@ template <typename T>
 __global__ void __launch_bounds__( 256 ) k_lu_diag( int mode, T* A, long lda, T* __restrict__ inv, int* check, T thr ) {
    typedef typename Mfma<T>::v4 v4;
    __shared__ T S[ kNB * kLd ];
+   __builtin_amdgcn_s_setprio( 3 );  // the head of the critical path: issue priority on its CU
    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, i = lane & 31;
    auto tile = [&]( int I, int J ) { return S + 32 * J * kLd + 32 * I; };
    {
