@@ -48,6 +48,21 @@ __device__ inline double block_sum( double v, double* red ) {
 // pair integrals of nearly aligned triangle pairs are sensitive to the last bits of the shifted
 // coordinates (an entry moved by 2e-8 relative when the centroid differed by 6e-15 A), so the
 // shift must be the reference's exactly.
+// run + t[0] + t[1] + ... + t[m - 1], added one at a time in that order (the running sum), with
+// the LDS reads of 16 terms issued ahead of their additions.
+__device__ inline double ordered_sum( double run, const double* t, int m ) {
+   int q = 0;
+   for ( ; q + 16 <= m; q += 16 ) {
+      double v[ 16 ];
+#pragma unroll
+      for ( int u = 0; u < 16; ++u ) v[ u ] = t[ q + u ];
+#pragma unroll
+      for ( int u = 0; u < 16; ++u ) run = run + v[ u ];
+   }
+   for ( ; q < m; ++q ) run = run + t[ q ];
+   return run;
+}
+
 __global__ void __launch_bounds__( 1024 ) k_prep( int n, double* v1, double* v2, double* v3, int iarea, double truearea,
                                                  double* area, double* scal /* [0]=afac [1]=totarea [2..4]=centroid */ ) {
    constexpr int kChunk = 1024;
@@ -59,10 +74,7 @@ __global__ void __launch_bounds__( 1024 ) k_prep( int n, double* v1, double* v2,
       if ( j < n )
          for ( int k = 0; k < 3; ++k ) term[ k * kChunk + threadIdx.x ] = ( v1[ k * n + j ] + v2[ k * n + j ] + v3[ k * n + j ] ) / 3.0;
       __syncthreads();
-      if ( threadIdx.x < 3 ) {
-         const int m = min( kChunk, n - j0 );
-         for ( int q = 0; q < m; ++q ) run = run + term[ threadIdx.x * kChunk + q ];
-      }
+      if ( threadIdx.x < 3 ) run = ordered_sum( run, term + threadIdx.x * kChunk, min( kChunk, n - j0 ) );
       __syncthreads();
    }
    if ( threadIdx.x < 3 ) shared_sum[ threadIdx.x ] = run / (double)n;
@@ -85,10 +97,7 @@ __global__ void __launch_bounds__( 1024 ) k_prep( int n, double* v1, double* v2,
          term[ threadIdx.x ] = ar;
       }
       __syncthreads();
-      if ( threadIdx.x == 0 ) {
-         const int m = min( kChunk, n - i0 );
-         for ( int q = 0; q < m; ++q ) tot = tot + term[ q ];
-      }
+      if ( threadIdx.x == 0 ) tot = ordered_sum( tot, term, min( kChunk, n - i0 ) );
       __syncthreads();
    }
    if ( threadIdx.x == 0 ) shared_sum[ 0 ] = tot;

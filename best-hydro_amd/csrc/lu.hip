@@ -1024,39 +1024,47 @@ __device__ inline void wave_atomic_max_nonneg( double* addr, double v ) {
    if ( ( threadIdx.x & 63 ) == 0 ) atomicMax( reinterpret_cast<unsigned long long*>( addr ), b );
 }
 
-// r = b - sum of the chunks (in chunk order); norms[r] = max |r(:, rhs)| over rows < n3.
+// r = b - sum of the chunks (in chunk order); norms[r] = max |r(:, rhs)| over rows < n3.  One
+// thread per row and right-hand side (blockIdx.y), so the chunk sums of all columns run at once.
 __global__ void __launch_bounds__( 256 ) k_resid_final( int npad, int n3, int nchunks, int nrhs,
                                                        const double* __restrict__ part, const double* __restrict__ b,
                                                        long ldb, double* __restrict__ r, long ldr, double* norms ) {
    const int row = blockIdx.x * 256 + threadIdx.x;
-   const bool in = row < npad;  // no early return: the wave reduction needs every lane
-   for ( int k = 0; k < nrhs; ++k ) {
-      double v = 0.0;
-      if ( in ) {
-         double s = 0.0;
-         for ( int c = 0; c < nchunks; ++c ) s = s + part[ (long)c * 16 * npad + (long)k * npad + row ];
-         v = row < n3 ? b[ (long)k * ldb + row ] - s : 0.0;
-         r[ (long)k * ldr + row ] = v;
+   const int k = blockIdx.y;
+   double v = 0.0;
+   if ( row < npad ) {  // no early return: the wave reduction needs every lane
+      const double* p = part + (long)k * npad + row;
+      const long cs = 16L * npad;
+      double s = 0.0;
+      int c = 0;
+      for ( ; c + 8 <= nchunks; c += 8 ) {  // eight loads in flight, added in chunk order
+         double t[ 8 ];
+#pragma unroll
+         for ( int u = 0; u < 8; ++u ) t[ u ] = p[ ( c + u ) * cs ];
+#pragma unroll
+         for ( int u = 0; u < 8; ++u ) s = s + t[ u ];
       }
-      wave_atomic_max_nonneg( &norms[ k ], fabs( v ) );
+      for ( ; c < nchunks; ++c ) s = s + p[ c * cs ];
+      v = row < n3 ? b[ (long)k * ldb + row ] - s : 0.0;
+      r[ (long)k * ldr + row ] = v;
    }
+   wave_atomic_max_nonneg( &norms[ k ], fabs( v ) );
 }
 
 // x += d (d in FP32 from the low-precision solve); norms[0..nrhs) = max|d|, norms[16..) = max|x|.
+// One thread per row and right-hand side (blockIdx.y).
 __global__ void __launch_bounds__( 256 ) k_ir_update( int n3, int nrhs, double* __restrict__ x, long ldx,
                                                      const float* __restrict__ d, long ldd, double* norms ) {
    const int row = blockIdx.x * 256 + threadIdx.x;
-   const bool in = row < n3;  // no early return: the wave reductions need every lane
-   for ( int k = 0; k < nrhs; ++k ) {
-      double dv = 0.0, xv = 0.0;
-      if ( in ) {
-         dv = (double)d[ (long)row * 16 + k ];
-         xv = x[ (long)k * ldx + row ] + dv;
-         x[ (long)k * ldx + row ] = xv;
-      }
-      wave_atomic_max_nonneg( &norms[ k ], fabs( dv ) );
-      wave_atomic_max_nonneg( &norms[ 16 + k ], fabs( xv ) );
+   const int k = blockIdx.y;
+   double dv = 0.0, xv = 0.0;
+   if ( row < n3 ) {  // no early return: the wave reductions need every lane
+      dv = (double)d[ (long)row * 16 + k ];
+      xv = x[ (long)k * ldx + row ] + dv;
+      x[ (long)k * ldx + row ] = xv;
    }
+   wave_atomic_max_nonneg( &norms[ k ], fabs( dv ) );
+   wave_atomic_max_nonneg( &norms[ 16 + k ], fabs( xv ) );
 }
 
 __global__ void k_d2f( long n, const double* __restrict__ a, float* __restrict__ b ) {
@@ -1420,14 +1428,14 @@ void residual( hipStream_t st, int npad, int n3, int nrhs, const double* A, long
    ProfScope prof( kKResid, 2.0 * npad * npad * nrhs, st );
    hipLaunchKernelGGL( k_resid_part, grid, dim3( 256 ), chunk * 16 * sizeof( double ), st, npad, chunk, nrhs, A, lda, x,
                        ldx, part );
-   hipLaunchKernelGGL( k_resid_final, dim3( ( npad + 255 ) / 256 ), dim3( 256 ), 0, st, npad, n3, nchunks, nrhs, part, b,
-                       ldb, r, ldr, norms );
+   hipLaunchKernelGGL( k_resid_final, dim3( ( npad + 255 ) / 256, nrhs ), dim3( 256 ), 0, st, npad, n3, nchunks, nrhs, part,
+                       b, ldb, r, ldr, norms );
 }
 
 size_t residual_part_elems( int npad ) { return (size_t)( npad / resid_chunk( npad ) ) * 16 * npad; }
 
 void ir_update( hipStream_t st, int n3, int nrhs, double* x, long ldx, const float* d, long ldd, double* norms ) {
-   hipLaunchKernelGGL( k_ir_update, dim3( ( n3 + 255 ) / 256 ), dim3( 256 ), 0, st, n3, nrhs, x, ldx, d, ldd, norms );
+   hipLaunchKernelGGL( k_ir_update, dim3( ( n3 + 255 ) / 256, nrhs ), dim3( 256 ), 0, st, n3, nrhs, x, ldx, d, ldd, norms );
 }
 
 void convert_d2f( hipStream_t st, long n, const double* a, float* b ) {
