@@ -49,6 +49,10 @@ struct LuStreams {
    hipStream_t st, st2, st3;
    hipEvent_t ev_u, ev_g;
    hipEvent_t ev[ kLuEvents ];
+   // optional: recorded on st when the trailing matrix first has fewer than late_rest rows (the
+   // factorization's latency-bound end), or at its end
+   hipEvent_t ev_late = nullptr;
+   int late_rest = 0;
 };
 
 // Kernel classes timed by the optional launch profiler.

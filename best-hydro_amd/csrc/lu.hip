@@ -1205,6 +1205,7 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
    bool pending_n = false;  // an update on st2 (part N) that the next pair's next-columns update must wait for
    bool used2 = false;      // st2 has work to join at the end
    bool deferred = false;   // the previous pair's part B is still to be applied (with this pair's)
+   bool late_done = false;  // S.ev_late recorded
    for ( int k0 = 0; k0 < nblk; k0 += 2 ) {
       const int k1 = k0 + 1;
       if ( k1 >= nblk ) {  // a last single block (npad an odd multiple of 128)
@@ -1214,6 +1215,10 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
       const bool last = k1 + 1 == nblk;
       const int c2 = k1 + 1, rest = npad - c2 * nb;
       const int nnext = std::min( 2 * nb, rest );
+      if ( S.ev_late && !late_done && npad - k0 * nb < S.late_rest ) {
+         BH_HIP( hipEventRecord( S.ev_late, st ) );
+         late_done = true;
+      }
       diag( k0 );
       if ( !last ) {  // U(k0, c2..) on st3, once part A of the previous update has reached those rows
          order( S.ev[ 0 ], st, st3 );
@@ -1290,6 +1295,7 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
    // the caller's later work on st must see all of st2's updates (st3's work is joined into st
    // above)
    if ( used2 ) BH_HIP( hipStreamWaitEvent( st, S.ev[ 6 ], 0 ) );
+   if ( S.ev_late && !late_done ) BH_HIP( hipEventRecord( S.ev_late, st ) );
 }
 template void lu_nopiv<double>( const LuStreams&, int, double*, long, double*, int* );
 template void lu_nopiv<float>( const LuStreams&, int, float*, long, float*, int* );

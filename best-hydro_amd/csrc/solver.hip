@@ -627,7 +627,8 @@ void prepare_buffers( bhg_solver* s ) {
 // with no host interaction: nothing here reads a device value, allocates or sets memory, so the
 // sequence can be captured in a HIP graph.  The pivot check and the refinement's convergence are
 // read afterwards (finish_enqueued).
-void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, int icov, const MeshTune& t ) {
+void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, int icov, const MeshTune& t,
+                    hipEvent_t late = nullptr ) {
    const int n = s->n;
    const long np = s->npad;
    g_gemm_bk_f32 = t.gemm_bk;
@@ -643,7 +644,10 @@ void enqueue_solve( bhg_solver* s, int iarea, double cg, int ivisco, int icov, c
    s->a32_fresh = false;
    fill( s->st, s->flags.p, 4, 0 );
    g_growth = s->fp32_growth;
-   lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
+   LuStreams ls = s->lu_streams();
+   ls.ev_late = late;  // the staggered series starts the next mesh here (half of the rows left)
+   ls.late_rest = s->npad / 2;
+   lu_nopiv<float>( ls, s->npad, s->a32.p, np, s->inv32.p, s->flags.p );
    g_growth = 1.0;
    build_super_inverses<float>( s->st, s->npad, kSuper, s->a32.p, np, s->inv32.p, s->xl32.p, s->xu32.p, s->tmp32.p );
    launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
@@ -1525,11 +1529,13 @@ struct bhg_series {
    std::vector<bhg_solver*> m;
    std::vector<bhg::MeshTune> tune;
    hipStream_t so = nullptr;
-   std::vector<hipEvent_t> ev;  // fork and join per mesh
+   std::vector<hipEvent_t> ev;       // fork and join per mesh
+   std::vector<hipEvent_t> ev_late;  // per mesh: its factorization's latency-bound end begins
    hipEvent_t t0 = nullptr, t1 = nullptr;
    hipGraph_t graph = nullptr;
    hipGraphExec_t exec = nullptr;
-   int parallel = 1;            // meshes side by side in the schedule (1) or one after another (0)
+   int parallel = 1;  // meshes side by side (1), one after another (0), or staggered (2): each mesh
+                      // starts when the one before it has half of its rows left to factor
    int key_iarea = -1, key_ivisco = -1, key_icov = -1;
    double key_cg = -1.0;
    std::string last_error;
@@ -1542,6 +1548,7 @@ struct bhg_series {
    ~bhg_series() {
       drop_graph();
       for ( auto e : ev ) (void)hipEventDestroy( e );
+      for ( auto e : ev_late ) (void)hipEventDestroy( e );
       if ( t0 ) (void)hipEventDestroy( t0 );
       if ( t1 ) (void)hipEventDestroy( t1 );
       if ( so ) (void)hipStreamDestroy( so );
@@ -1555,7 +1562,10 @@ namespace bhg {
 namespace {
 
 // Captures the solves of every mesh (enqueue_solve) as one HIP graph: forked from the series
-// stream to each mesh's streams and joined back, side by side or chained one after another.
+// stream to each mesh's streams and joined back, side by side, chained one after another, or
+// staggered: mesh k starts when mesh k - 1's factorization reaches its latency-bound end (half
+// of its rows left), so that the next mesh's assembly and early, GEMM-bound factorization fill
+// what the previous one's chain of small launches leaves idle.
 void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int icov, int parallel ) {
    BH_HIP( hipSetDevice( S->device ) );
    S->drop_graph();
@@ -1567,9 +1577,13 @@ void series_capture( bhg_series* S, int iarea, double cg, int ivisco, int icov, 
       hipStream_t prev = S->so;
       for ( int k = 0; k < nm; ++k ) {
          bhg_solver* m = S->m[ k ];
-         BH_HIP( hipEventRecord( S->ev[ 2 * k ], parallel ? S->so : prev ) );
-         BH_HIP( hipStreamWaitEvent( m->st, S->ev[ 2 * k ], 0 ) );
-         enqueue_solve( m, iarea, cg, ivisco, icov, S->tune[ k ] );
+         if ( parallel == 2 && k > 0 ) {
+            BH_HIP( hipStreamWaitEvent( m->st, S->ev_late[ k - 1 ], 0 ) );
+         } else {
+            BH_HIP( hipEventRecord( S->ev[ 2 * k ], parallel ? S->so : prev ) );
+            BH_HIP( hipStreamWaitEvent( m->st, S->ev[ 2 * k ], 0 ) );
+         }
+         enqueue_solve( m, iarea, cg, ivisco, icov, S->tune[ k ], parallel == 2 ? S->ev_late[ k ] : nullptr );
          BH_HIP( hipEventRecord( S->ev[ 2 * k + 1 ], m->st ) );
          prev = m->st;
       }
@@ -1631,6 +1645,8 @@ bhg_series* bhg_series_create( int device, int nmesh ) {
       BH_HIP( hipEventCreate( &S->t1 ) );
       S->ev.resize( 2 * nmesh, nullptr );
       for ( auto& e : S->ev ) BH_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+      S->ev_late.resize( nmesh, nullptr );
+      for ( auto& e : S->ev_late ) BH_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
       for ( int k = 0; k < nmesh; ++k ) {
          bhg_solver* m = bhg_create( device );
          if ( !m ) throw std::runtime_error( "bhg_create" );
@@ -1765,7 +1781,7 @@ int bhg_series_autotune( bhg_series* S, int iarea, double cg, int ivisco ) {
       int choice = 1, choice_k = 0;
       for ( int kset : { 0, 1 } ) {
          S->tune = kset ? k32 : tuned;
-         for ( int par : { 1, 0 } ) {
+         for ( int par : { 1, 0, 2 } ) {
             bhg::series_capture( S, iarea, cg, ivisco, 0, par );
             bhg::series_replay( S );
             const double ms = bhg::series_replay( S );

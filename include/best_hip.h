@@ -275,16 +275,17 @@ int bhg_series_set_mesh_file( bhg_series* s, int k, const char* path, int iarea 
 
 /* The autotuner: for each mesh, the pair kernel's strip of source triangles per workgroup (16, 32,
  * 64, 128) and the FP32 GEMM's K slice (16, 32, 64) that time fastest, the refinement steps an
- * eager solve needed; then, by timing four graphs, whether the meshes run side by side or one
- * after another and whether they keep those K slices or all take 32 (what is fastest for a mesh
- * alone can be slower when the meshes share the GPU).  Captures the schedule. */
+ * eager solve needed; then, by timing six graphs, whether the meshes run side by side, one after
+ * another, or staggered (each starts when the one before it has half of its rows left to factor),
+ * and whether they keep those K slices or all take 32 (what is fastest for a mesh alone can be
+ * slower when the meshes share the GPU).  Captures the schedule. */
 int bhg_series_autotune( bhg_series* s, int iarea, double cg, int ivisco );
 /* gemm_bk: 16, 32 or 64 on every tile, or 0 (a single solve's default): 16 on the 128 x 128 tiles
  * of the trailing updates and 32 on the smaller ones. */
 int bhg_series_set_tuning( bhg_series* s, int k, int strip, int gemm_bk, int ir_iters );
 /* bhg_set_option for every mesh of the series (the schedule is captured again). */
 int bhg_series_set_option( bhg_series* s, const char* name, double value );
-/* out[0..3] = strip, gemm_bk, ir_iters, side by side (1) or chained (0). */
+/* out[0..3] = strip, gemm_bk, ir_iters, side by side (1), chained (0) or staggered (2). */
 int bhg_series_get_tuning( bhg_series* s, int k, int* out );
 
 /* One launch of the captured schedule (captured first if the options changed), then per mesh: the
