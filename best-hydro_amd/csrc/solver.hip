@@ -454,6 +454,9 @@ bool ir_loop( bhg_solver* s, int nrhs, int it0, int maxit ) {
    double prev = 1e300, worst = 1e300;
    for ( int it = it0; it < maxit; ++it ) {
       ir_step( s, nrhs, it );
+      // step 0's correction is the whole of x0 and decides nothing: the next step follows
+      // without a host round trip (its norms stay on the device, where step 1's estimate reads them)
+      if ( it == 0 && it + 1 < maxit ) continue;
       const IrCorrection c = ir_correction( s, nrhs );
       worst = c.worst;
       s->stats.ir_iterations = it + 1;
