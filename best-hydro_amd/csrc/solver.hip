@@ -70,7 +70,7 @@ constexpr double kFp32Growth = 256.0;
 // Order of the diagonal super-blocks whose inverses the triangular solves use; the matrix is
 // padded to a multiple of it.
 constexpr int kSuper = 1024;
-constexpr int kPairStrip = 64;  // source triangles per workgroup of the pair kernel
+constexpr int kPairStrip = 16;  // source triangles per workgroup of the pair kernel
 
 struct PhaseTimer {
    hipEvent_t ev[ 8 ] = {};
@@ -159,6 +159,14 @@ struct bhg_solver {
                                            // refactorization (0: ir_maxit)
    int host_algebra = 0;                   // 1: the reductions summed and BEST's algebra done on the
                                            // host (a check of the device algebra)
+   // The refinement steps the last FP32 dense solve of the resident mesh took, and the arguments
+   // its matrix and right-hand sides came from: a solve with the same ones enqueues that many
+   // steps before its first host check (0: none known; any setter that changes the matrix or the
+   // refinement clears it).  ir_ahead is the count in force for the solve running now.
+   int ir_hint = 0, hint_iarea = 0, hint_nrhs = 0;
+   double hint_cg = 0.0;
+   int ir_ahead = 0;
+   bool flag_pending = false;  // the FP32 LU's multiplier check is read at the first host check
    const bhg::SolveWork* work_override = nullptr;  // solve_device's work vectors when not the buffers above
 
    void say( const char* fmt, ... ) __attribute__( ( format( printf, 2, 3 ) ) ) {
@@ -284,9 +292,12 @@ int read_flag( bhg_solver* s ) {
 
 // Factors the assembled matrix: precision 64 in place on a64 (GR is consumed), 32 on an FP32
 // copy (a64 is kept for the refinement residuals).  Partial pivoting falls back to the slow path
-// when the exchange-free factorization is not GEPP's (see lu.hip).
-void factor( bhg_solver* s, int precision ) {
+// when the exchange-free factorization is not GEPP's (see lu.hip).  With defer_flag (FP32) the
+// multiplier check is not read here: solve_device reads it at the refinement's first host check
+// and refactors with row exchanges then.
+void factor( bhg_solver* s, int precision, bool defer_flag = false ) {
    s->hm.reset();
+   s->flag_pending = false;
    const size_t np = s->npad;
    const int nblk = s->npad / kNB;
    s->ipiv.ensure( np );
@@ -316,7 +327,9 @@ void factor( bhg_solver* s, int precision ) {
       g_growth = s->fp32_growth;
       lu_nopiv<float>( s->lu_streams(), s->npad, s->a32.p, (long)np, s->inv32.p, s->flags.p );
       g_growth = 1.0;
-      if ( read_flag( s ) ) {
+      if ( defer_flag ) {
+         s->flag_pending = true;
+      } else if ( read_flag( s ) ) {
          s->say( "LU (FP32): a multiplier exceeded %g; refactoring with row exchanges", s->fp32_growth );
          convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
          fill( s->st, s->flags.p, 4, 0 );
@@ -450,18 +463,26 @@ bool ir_converged( const IrCorrection& c, int it, double tol ) { return it > 0 &
 // FP64 residuals themselves then limit it; the log says so and bhg_stats.ir_above_tol is set.
 // Returns false if it stopped above kIrFloor (the FP32 factors too poor for the refinement).
 constexpr double kIrFloor = 1e-10;
-bool ir_loop( bhg_solver* s, int nrhs, int it0, int maxit ) {
+enum IrOutcome { kIrConverged, kIrFailed, kIrFlagged };
+IrOutcome ir_run( bhg_solver* s, int nrhs, int it0, int maxit ) {
    double prev = 1e300, worst = 1e300;
+   // Host checks begin after step `first`: step 0's correction is the whole of x0 and decides
+   // nothing, so step 1 follows it without a round trip (its norms stay on the device, where step
+   // 1's estimate reads them); a solve that knows how many steps its matrix needs (ir_ahead)
+   // enqueues them all first.
+   const int first = std::min( maxit, it0 + ( it0 == 0 ? std::max( 2, s->ir_ahead ) : 1 ) ) - 1;
    for ( int it = it0; it < maxit; ++it ) {
       ir_step( s, nrhs, it );
-      // step 0's correction is the whole of x0 and decides nothing: the next step follows
-      // without a host round trip (its norms stay on the device, where step 1's estimate reads them)
-      if ( it == 0 && it + 1 < maxit ) continue;
+      if ( it < first ) continue;
+      if ( s->flag_pending ) {  // before any correction is read: flagged factors may give non-finite ones
+         s->flag_pending = false;
+         if ( read_flag( s ) ) return kIrFlagged;
+      }
       const IrCorrection c = ir_correction( s, nrhs );
       worst = c.worst;
       s->stats.ir_iterations = it + 1;
       s->stats.ir_correction = worst;
-      if ( ir_converged( c, it, s->ir_tol ) ) return true;
+      if ( ir_converged( c, it, s->ir_tol ) ) return kIrConverged;
       if ( it > 2 && worst > 0.5 * prev ) break;  // stagnation
       prev = worst;
    }
@@ -469,17 +490,25 @@ bool ir_loop( bhg_solver* s, int nrhs, int it0, int maxit ) {
       s->stats.ir_above_tol = 1;
       s->say( "refinement: stagnated at a correction of %.2e, above the tolerance %.0e; accepted (below %.0e)", worst,
               s->ir_tol, kIrFloor );
-      return true;
+      return kIrConverged;
    }
-   return false;
+   return kIrFailed;
+}
+bool ir_loop( bhg_solver* s, int nrhs, int it0, int maxit ) {
+   const IrOutcome o = ir_run( s, nrhs, it0, maxit );
+   if ( o == kIrFlagged ) throw std::runtime_error( "refinement: the multiplier check was still pending" );
+   return o == kIrConverged;
 }
 bool ir_loop( bhg_solver* s, int nrhs, int it0 ) { return ir_loop( s, nrhs, it0, s->ir_maxit ); }
 
 // The FP32 factors of the resident (deflated) matrix with row exchanges, after the exchange-free
 // ones did not let the refinement converge.
-void refactor_pivoted32( bhg_solver* s ) {
+void refactor_pivoted32( bhg_solver* s, bool flagged = false ) {
    const size_t np = s->npad;
-   s->say( "LU (FP32): refinement did not converge with the exchange-free factors; refactoring with row exchanges" );
+   if ( flagged )
+      s->say( "LU (FP32): a multiplier exceeded %g; refactoring with row exchanges", s->fp32_growth );
+   else
+      s->say( "LU (FP32): refinement did not converge with the exchange-free factors; refactoring with row exchanges" );
    convert_d2f( s->st, (long)( np * np ), s->a64.p, s->a32.p );
    fill( s->st, s->flags.p, 4, 0 );
    lu_pivoted<float>( s->st, s->npad, s->a32.p, (long)np, s->inv32.p, s->ipiv.p, s->flags.p + 1, s->pval.p, s->pidx.p );
@@ -511,9 +540,13 @@ void solve_device( bhg_solver* s, int nrhs ) {
    }
    if ( s->factored != 32 ) throw std::runtime_error( "solve before the factorization" );
    const int first = s->ir_steps_nopiv > 0 && !s->pivoted ? std::min( s->ir_steps_nopiv, s->ir_maxit ) : s->ir_maxit;
-   if ( ir_loop( s, nrhs, 0, first ) ) return;
-   if ( s->pivoted || s->transposed ) throw std::runtime_error( "iterative refinement did not converge" );
-   refactor_pivoted32( s );
+   const IrOutcome o = ir_run( s, nrhs, 0, first );
+   if ( o == kIrConverged ) return;
+   if ( o == kIrFailed && ( s->pivoted || s->transposed ) ) throw std::runtime_error( "iterative refinement did not converge" );
+   // the exchange-free factors had a multiplier above the threshold (read only now), or did not
+   // let the refinement converge: the same matrix with row exchanges
+   refactor_pivoted32( s, o == kIrFlagged );
+   s->ir_ahead = 0;
    if ( !ir_loop( s, nrhs, 0 ) ) throw std::runtime_error( "iterative refinement did not converge" );
 }
 
@@ -706,6 +739,7 @@ void solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int 
    const int n = s->n;
    const long np = s->npad;
    s->cg = cg;
+   s->flag_pending = false;
    struct ProfBind {
       explicit ProfBind( LaunchProfiler* p ) { g_prof = p->on ? p : nullptr; }
       ~ProfBind() { g_prof = nullptr; }
@@ -724,13 +758,22 @@ void solve_mesh( bhg_solver* s, int iarea, double cg, int ivisco, int icov, int 
    } else {
       assemble( s, true, precision == 32 );
       T.mark( s->st, 2 );
-      factor( s, precision );
+      factor( s, precision, precision == 32 );
    }
    T.mark( s->st, 3 );
    size_vectors( s );
    const int nrhs = ivisco ? 11 : 6;
+   const bool dense32 = s->method == 0 && precision == 32;
+   s->ir_ahead = dense32 && s->hint_iarea == iarea && s->hint_cg == cg && s->hint_nrhs == nrhs ? s->ir_hint : 0;
    launch_rhs( s->st, n, s->xc.p, s->b.p, np, ivisco ? 1 : 0 );
    solve_device( s, nrhs );
+   s->ir_ahead = 0;
+   if ( dense32 ) {
+      s->ir_hint = s->pivoted ? 0 : s->stats.ir_iterations;
+      s->hint_iarea = iarea;
+      s->hint_cg = cg;
+      s->hint_nrhs = nrhs;
+   }
    T.mark( s->st, 4 );
    compute_results( s, iarea, cg, ivisco, icov, out );
    T.mark( s->st, 5 );
@@ -1115,6 +1158,7 @@ int bhg_set_mesh( bhg_solver* s, int n, const double* v1, const double* v2, cons
       s->mesh_raw = true;
       s->mesh_ready = false;
       s->factored = 0;
+      s->ir_hint = 0;
    } );
 }
 
@@ -1194,12 +1238,14 @@ int bhg_set_refinement( bhg_solver* s, double tol, int maxit ) {
    if ( !s ) return 1;
    s->ir_tol = tol;
    s->ir_maxit = maxit;
+   s->ir_hint = 0;
    return 0;
 }
 
 int bhg_set_option( bhg_solver* s, const char* name, double value ) {
    if ( !s || !name ) return 1;
    const std::string k( name );
+   s->ir_hint = 0;
    if ( k == "fp32_growth" && value >= 1.0 ) {
       s->fp32_growth = value;
    } else if ( k == "ir_steps_before_pivoting" && value >= 0.0 ) {
@@ -1216,6 +1262,7 @@ int bhg_set_option( bhg_solver* s, const char* name, double value ) {
 int bhg_set_method( bhg_solver* s, int method, double eps, int single, double tol ) {
    if ( !s || ( method != 0 && method != 1 ) ) return 1;
    s->method = method;
+   s->ir_hint = 0;
    s->hm_opt.eps = eps > 0.0 ? eps : 1e-6;
    s->hm_opt.single = single ? 1 : 0;
    s->hm_opt.tol = tol;
