@@ -19,6 +19,10 @@ constexpr int kNB = 128;
 
 // Right-hand sides per solve call at most.
 constexpr int kMaxRhs = 16;
+// Columns of K per workgroup of the triangular solves' split-K products (k_skinny_part), and the
+// most slices one product has: its K is at most a super-block order of 1024.
+constexpr int kSkinnySlice = 128;
+constexpr int kSkinnyParts = 1024 / kSkinnySlice;
 
 // Per-source-triangle record used by the pair kernel: the vertices, the corrected area and the
 // source-only quantities of the quadrature (edge vectors, their dot products, sqrt(d2121)).

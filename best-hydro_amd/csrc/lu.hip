@@ -1328,7 +1328,6 @@ void lu_pivoted( hipStream_t st, int npad, T* A, long lda, T* inv, int* ipiv, in
 template void lu_pivoted<double>( hipStream_t, int, double*, long, double*, int*, int*, double*, int* );
 template void lu_pivoted<float>( hipStream_t, int, float*, long, float*, int*, int*, double*, int* );
 
-constexpr int kSkinnySlice = 256;  // columns of K per workgroup of k_skinny_part
 
 // Y = beta Y + alpha A X (A column-major M x K, X and Y row-interleaved); P is scratch of
 // (K / kSkinnySlice) M 16 elements.  M must be a multiple of 64, K of kSkinnySlice.

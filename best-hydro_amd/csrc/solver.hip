@@ -70,6 +70,7 @@ constexpr double kFp32Growth = 256.0;
 // Order of the diagonal super-blocks whose inverses the triangular solves use; the matrix is
 // padded to a multiple of it.
 constexpr int kSuper = 1024;
+static_assert( kSuper / kSkinnySlice <= kSkinnyParts, "the triangular solves' split-K partials of one super-block" );
 constexpr int kPairStrip = 16;  // source triangles per workgroup of the pair kernel
 
 struct PhaseTimer {
@@ -213,14 +214,14 @@ __global__ void k_ir_shift( double* norms ) {
 // bhg_besolv_noalloc: every work vector of one solve (the FP64 right-hand sides, solutions and
 // residuals, the split-K partial sums of the triangular solves in both precisions, the residual
 // product's partial sums, the refinement's norms) for matrices of order up to kNoAllocMaxPad and
-// up to kMaxRhs right-hand sides, in one file-scope device array (89 MiB on each GPU the library
+// up to kMaxRhs right-hand sides, in one file-scope device array (101 MiB on each GPU the library
 // runs on, held until the process ends).  Host code reaches it only through the address
 // hipGetSymbolAddress returns on each device (noalloc_scratch below).
 constexpr long kNoAllocMaxPad = 16384;
 constexpr long kNaVec = kNoAllocMaxPad * kMaxRhs;                      // one npad x 16 block
 constexpr long kNaPart = ( kNoAllocMaxPad / 512 ) * 16 * kNoAllocMaxPad;  // residual partials at that order
-constexpr size_t kNaDoubles = 9 * (size_t)kNaVec + (size_t)kNaPart + 64;  // b x r y64 x64il, p64 (4), part, norms
-constexpr size_t kNaFloats = 7 * (size_t)kNaVec;                          // d32 y32 x32, p32 (4)
+constexpr size_t kNaDoubles = ( 5 + kSkinnyParts ) * (size_t)kNaVec + (size_t)kNaPart + 64;  // b x r y64 x64il, p64, part, norms
+constexpr size_t kNaFloats = ( 3 + kSkinnyParts ) * (size_t)kNaVec;                          // d32 y32 x32, p32
 constexpr size_t kNaBytes = ( ( kNaDoubles * sizeof( double ) + kNaFloats * sizeof( float ) ) + 255 ) / 256 * 256;
 static __device__ __align__( 256 ) char scratch[ kNaBytes ];
 
@@ -383,11 +384,11 @@ void size_vectors( bhg_solver* s ) {
    s->r.ensure( need );
    s->y64.ensure( need );
    s->x64il.ensure( need );
-   s->p64.ensure( need * 4 );
+   s->p64.ensure( need * kSkinnyParts );
    s->d32.ensure( need );
    s->y32.ensure( need );
    s->x32.ensure( need );
-   s->p32.ensure( need * 4 );
+   s->p32.ensure( need * kSkinnyParts );
    s->part.ensure( residual_part_elems( s->npad ) );
    s->norms.ensure( 64 );
    s->sums.ensure( (size_t)kSumGroups * kSumSlots );
@@ -1551,7 +1552,7 @@ int bhg_besolv_noalloc( bhg_solver* s, int nrhs, const double* d_b, long ldb, do
       bhg::SolveWork w {};
       w.b = d, w.x = d + bhg::kNaVec, w.r = d + 2 * bhg::kNaVec, w.y64 = d + 3 * bhg::kNaVec, w.x64il = d + 4 * bhg::kNaVec;
       w.p64 = d + 5 * bhg::kNaVec;
-      w.part = d + 9 * bhg::kNaVec;
+      w.part = d + ( 5 + bhg::kSkinnyParts ) * bhg::kNaVec;
       w.norms = w.part + bhg::kNaPart;
       float* f = reinterpret_cast<float*>( base + bhg::kNaDoubles * sizeof( double ) );
       w.d32 = f, w.y32 = f + bhg::kNaVec, w.x32 = f + 2 * bhg::kNaVec, w.p32 = f + 3 * bhg::kNaVec;

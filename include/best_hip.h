@@ -175,7 +175,7 @@ int bhg_besolv( bhg_solver* s, int nrhs, double* b, long ldb );
 /* bhg_besolv for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  b and x are
  * DEVICE pointers (3n x nrhs, leading dimensions ldb and ldx); the solve's work vectors live in
- * one static device array of the library (89 MiB on each GPU it runs on, held until the process
+ * one static device array of the library (101 MiB on each GPU it runs on, held until the process
  * ends), so nothing is allocated or freed: the resident factors of bhg_solve_mesh or
  * bhg_factor_deflated (FP64 LU, or FP32 LU + FP64 refinement) are used as they are.  Limits: padded
  * matrix order up to 16,384 (5,461 triangles) and up to 16 right-hand sides, else nonzero with a
