@@ -48,7 +48,7 @@ class BhResult(C.Structure):
         for name, _ in self._fields_:
             v = getattr(self, name)
             if isinstance(v, C.Array):
-                a = np.array(v[:], dtype=np.float64)
+                a = np.frombuffer(v, dtype=np.float64).copy()  # through the buffer protocol, no list
                 out[name] = a.reshape(3, 3, order="F") if len(a) == 9 else a
             else:
                 out[name] = v

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__( 256 ) k_sums( int n, const double* __restrict
                                                 const double* __restrict__ b, long ldb, int nstrain, double d1, double d2,
                                                 double d3, const double* __restrict__ dptr, int want_bb,
                                                 double* __restrict__ partial ) {
-   __shared__ double red[ 256 ];
+   __shared__ double red[ 4 * kSumSlots ];
    double acc[ kSumSlots ];
    if ( dptr ) d1 = dptr[ 0 ], d2 = dptr[ 1 ], d3 = dptr[ 2 ];  // the centre computed on the device
    for ( int k = 0; k < kSumSlots; ++k ) acc[ k ] = 0.0;
@@ -144,16 +144,20 @@ __global__ void __launch_bounds__( 256 ) k_sums( int n, const double* __restrict
 #undef H_
 #undef G_
    }
-   // Fixed-order tree reduction over the workgroup, slot by slot.
+   // Fixed-order reduction over the workgroup: each slot by a butterfly within each wave (the same
+   // lanes added in the same order every time), the four waves' sums added in wave order after one
+   // barrier.
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    for ( int k = 0; k < kSumSlots; ++k ) {
-      red[ threadIdx.x ] = acc[ k ];
-      __syncthreads();
-      for ( int w = 128; w > 0; w >>= 1 ) {
-         if ( threadIdx.x < w ) red[ threadIdx.x ] = red[ threadIdx.x ] + red[ threadIdx.x + w ];
-         __syncthreads();
-      }
-      if ( threadIdx.x == 0 ) partial[ (long)blockIdx.x * kSumSlots + k ] = red[ 0 ];
-      __syncthreads();
+      double v = acc[ k ];
+#pragma unroll
+      for ( int o = 32; o > 0; o >>= 1 ) v = v + __shfl_xor( v, o, 64 );
+      if ( lane == 0 ) red[ wave * kSumSlots + k ] = v;
+   }
+   __syncthreads();
+   if ( threadIdx.x < kSumSlots ) {
+      const int k = threadIdx.x;
+      partial[ (long)blockIdx.x * kSumSlots + k ] = ( red[ k ] + red[ kSumSlots + k ] ) + ( red[ 2 * kSumSlots + k ] + red[ 3 * kSumSlots + k ] );
    }
 }
 
