@@ -83,6 +83,11 @@ struct GemmProb {
    long ldc;
    int* check;
    T thr;
+   // The K range of each tile when an operand is triangular (the rest of its K is zeros, whose
+   // products are skipped: the sums keep their bits): 0 all of K; 1 from the tile's first column
+   // (B lower triangular); 2 to its last row (A lower); 3 to its last column (B upper); 4 from its
+   // first row (A upper).
+   int kmode = 0;
 };
 
 constexpr int kGemmPad = 8;
@@ -109,8 +114,10 @@ __device__ inline void gemm_tile( int bm, int bn, const GemmProb<T>& g, T* smem 
    T* C = g.C;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave & 1, wn = wave >> 1;
-   const T* Ab = g.A + (long)bm * BM;
-   const T* Bb = g.B + (long)bn * BN * ldb;
+   const int kb = g.kmode == 1 ? bn * BN : g.kmode == 4 ? bm * BM : 0;
+   const int ke = g.kmode == 2 ? min( K, ( bm + 1 ) * BM ) : g.kmode == 3 ? min( K, ( bn + 1 ) * BN ) : K;
+   const T* Ab = g.A + (long)bm * BM + (long)kb * lda;
+   const T* Bb = g.B + (long)bn * BN * ldb + kb;
    constexpr int AE = BM * BK / 256;  // A elements per thread, contiguous along m
    constexpr int ATPK = BM / AE;      // threads per k column of A
    constexpr int BE = BK * BN / 256;  // B elements per thread, contiguous along k
@@ -135,7 +142,7 @@ __device__ inline void gemm_tile( int bm, int bn, const GemmProb<T>& g, T* smem 
    // reads C in the epilogue.
    constexpr bool kPre = sizeof( T ) == 4;
    T cpre[ TM ][ TN ][ 4 ];
-   const int nk = K / BK;
+   const int nk = ( ke - kb ) / BK;
    auto gload = [&]( int kt ) {
       const T* pa = Ab + (long)( kt * BK + a_k ) * lda + a_m;
 #pragma unroll
@@ -205,7 +212,7 @@ __device__ inline void gemm_tile( int bm, int bn, const GemmProb<T>& g, T* smem 
 template <typename T, int BK, int BM = 128, int BN = 128>
 __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm( int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
                                                 T beta, T* C, long ldc, int* check, T thr, int zdiv, BatchStride sa,
-                                                BatchStride sb, BatchStride sc ) {
+                                                BatchStride sb, BatchStride sc, int kmode ) {
    __shared__ __attribute__( ( aligned( 16 ) ) ) T smem[ gemm_lds_elems<BK, BM, BN>() ];
    // the smaller tiles carry the factorization's critical path: their waves take issue
    // priority over the wide trailing updates' waves sharing a CU
@@ -231,7 +238,7 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( G
       bm = first + ( id % per ) % gm;
       bn = ( id % per ) / gm;
    }
-   gemm_tile<T, BK, BM, BN>( bm, bn, GemmProb<T> { M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, thr }, smem );
+   gemm_tile<T, BK, BM, BN>( bm, bn, GemmProb<T> { M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, thr, kmode }, smem );
 }
 
 // Two independent products in one launch, for the factorization's critical path: p0 on 64 x 128
@@ -1086,20 +1093,20 @@ thread_local double g_growth = 1.0;
 
 template <typename T, int BM, int BN>
 void launch_gemm_shape( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta, T* C,
-                        long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb, BatchStride sc ) {
+                        long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb, BatchStride sc, int kmode ) {
    dim3 grid( M / BM, N / BN, batch );
    if constexpr ( sizeof( T ) == 8 )
       hipLaunchKernelGGL( ( k_gemm<double, 16, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                          check, T( g_growth ), zdiv, sa, sb, sc );
+                          check, T( g_growth ), zdiv, sa, sb, sc, kmode );
    else if ( g_gemm_bk_f32 == 16 || K % 32 != 0 || ( g_gemm_bk_f32 == 0 && BM == 128 && BN == 128 ) )
       hipLaunchKernelGGL( ( k_gemm<float, 16, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                          check, T( g_growth ), zdiv, sa, sb, sc );
+                          check, T( g_growth ), zdiv, sa, sb, sc, kmode );
    else if ( g_gemm_bk_f32 == 64 && K % 64 == 0 )
       hipLaunchKernelGGL( ( k_gemm<float, 64, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                          check, T( g_growth ), zdiv, sa, sb, sc );
+                          check, T( g_growth ), zdiv, sa, sb, sc, kmode );
    else
       hipLaunchKernelGGL( ( k_gemm<float, 32, BM, BN> ), grid, dim3( 256 ), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                          check, T( g_growth ), zdiv, sa, sb, sc );
+                          check, T( g_growth ), zdiv, sa, sb, sc, kmode );
 }
 
 // Tiles of 128 x 128 where they fill the chip; the small products of the factorization's critical
@@ -1109,7 +1116,7 @@ void launch_gemm_shape( hipStream_t st, int M, int N, int K, T alpha, const T* A
 template <typename T>
 void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta,
                           T* C, long ldc, int* check, int batch, int zdiv, BatchStride sa, BatchStride sb,
-                          BatchStride sc ) {
+                          BatchStride sc, int kmode = 0 ) {
    if ( M <= 0 || N <= 0 || batch <= 0 ) return;
    const long tiles = (long)( M / 128 ) * ( N / 128 ) * batch;
    constexpr long kSmall = 256;  // CUs
@@ -1117,13 +1124,13 @@ void launch_gemm_batched( hipStream_t st, int M, int N, int K, T alpha, const T*
    ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : shape == 0 ? kKGemm32T : shape == 3 ? kKGemm32S : kKGemm32,
                    2.0 * M * N * K * batch, st );
    if ( shape == 1 )
-      launch_gemm_shape<T, 64, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+      launch_gemm_shape<T, 64, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc, kmode );
    else if ( shape == 2 )
-      launch_gemm_shape<T, 128, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+      launch_gemm_shape<T, 128, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc, kmode );
    else if ( shape == 3 )
-      launch_gemm_shape<T, 64, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+      launch_gemm_shape<T, 64, 64>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc, kmode );
    else
-      launch_gemm_shape<T, 128, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc );
+      launch_gemm_shape<T, 128, 128>( st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, check, batch, zdiv, sa, sb, sc, kmode );
 }
 
 // k_gemm_pair with the K slice launch_gemm_shape would choose for these tile shapes.
@@ -1346,7 +1353,8 @@ void launch_skinny( hipStream_t st, int M, int K, T alpha, const T* A, long lda,
 // inverses by recursive doubling: with A, B the inverses of two adjacent s x s diagonal blocks and
 // C the block between them, inv([A 0; C B]) has -B^-1 C A^-1 below (and inv of upper: -A^-1 C B^-1
 // right of the diagonal).  Every level is two batched GEMMs per triangle over all pairs of all
-// super-blocks.  tmp holds npad * S / 2 elements.
+// super-blocks; each product skips the K range where its inverse operand is zero (the
+// triangles of A^-1 and B^-1).  tmp holds npad * S / 2 elements.
 template <typename T>
 void build_super_inverses( hipStream_t st, int npad, int S, const T* LU, long lda, const T* inv, T* XL, T* XU, T* tmp ) {
    const int nsb = npad / S;
@@ -1359,16 +1367,16 @@ void build_super_inverses( hipStream_t st, int npad, int S, const T* LU, long ld
       // L: T = C A^-1, X21 = -B^-1 T
       BatchStride c_l { (long)S + (long)S * lda, 2L * s + 2L * s * lda };
       BatchStride a_x { SS, 2L * s * ( S + 1 ) };
-      launch_gemm_batched<T>( st, s, s, s, T( 1 ), LU + s, lda, XL, S, T( 0 ), tmp, s, nullptr, batch, zdiv, c_l, a_x, t_str );
+      launch_gemm_batched<T>( st, s, s, s, T( 1 ), LU + s, lda, XL, S, T( 0 ), tmp, s, nullptr, batch, zdiv, c_l, a_x, t_str, 1 );
       BatchStride x21 { SS, 2L * s * ( S + 1 ) };
       launch_gemm_batched<T>( st, s, s, s, T( -1 ), XL + (long)s * ( S + 1 ), S, tmp, s, T( 0 ), XL + s, S, nullptr, batch, zdiv,
-                              a_x, t_str, x21 );
+                              a_x, t_str, x21, 2 );
       // U: T = C B^-1, X12 = -A^-1 T
       BatchStride c_u { (long)S + (long)S * lda, 2L * s + 2L * s * lda };
       launch_gemm_batched<T>( st, s, s, s, T( 1 ), LU + (long)s * lda, lda, XU + (long)s * ( S + 1 ), S, T( 0 ), tmp, s, nullptr,
-                              batch, zdiv, c_u, a_x, t_str );
+                              batch, zdiv, c_u, a_x, t_str, 3 );
       launch_gemm_batched<T>( st, s, s, s, T( -1 ), XU, S, tmp, s, T( 0 ), XU + (long)s * S, S, nullptr, batch, zdiv, a_x, t_str,
-                              x21 );
+                              x21, 4 );
    }
 }
 template void build_super_inverses<double>( hipStream_t, int, int, const double*, long, const double*, double*, double*,
