@@ -7,7 +7,8 @@
 //                  its triangles inverted in place (LAPACK dtrti2 order);
 //   k_gemm         128 x 128 MFMA tiles (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32,
 //                  FP32 accumulation): the panel below (A U^-1), the row panel right (L^-1 A) and
-//                  the trailing update (A -= L U).
+//                  the trailing update (A -= L U); smaller tiles for the small products, and
+//                  k_gemm_pair / k_gemm_multi for two to four products in one launch.
 // GR from BEST's meshes is column diagonally dominant in the elimination sense: partial
 // pivoting never exchanges rows (every multiplier |l_ij| <= 1).  The fast path therefore
 // factors without exchanges and checks exactly that condition (|a_ij| <= |a_jj| at the moment
@@ -254,6 +255,27 @@ __global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( G
       gemm_tile<T, BK, 64, 128>( blockIdx.x, 0, p0, smem );
    else
       gemm_tile<T, BK, 128, 64>( 0, blockIdx.x - t0, p1, smem );
+}
+
+// Up to four independent products on 64 x 64 tiles in one launch (the narrow parts of a pair's
+// trailing update: one dependent launch instead of two or four on the update's stream).
+// Workgroups 0 .. end[0] - 1 take problem 0's tiles, the next ones problem 1's, and so on.
+template <typename T>
+struct GemmMulti {
+   GemmProb<T> p[ 4 ];
+   int end[ 4 ];  // running tile counts
+   int n;
+};
+
+template <typename T, int BK>
+__global__ void __launch_bounds__( 256 ) __attribute__( ( amdgpu_waves_per_eu( GemmOcc<T, BK>::waves ) ) ) k_gemm_multi( GemmMulti<T> mp ) {
+   __shared__ __attribute__( ( aligned( 16 ) ) ) T smem[ gemm_lds_elems<BK, 64, 64>() ];
+   __builtin_amdgcn_s_setprio( 2 );  // the next pair's chain waits for these (as the smaller tiles of k_gemm)
+   const int b = blockIdx.x;
+   int q = 0;
+   while ( q + 1 < mp.n && b >= mp.end[ q ] ) ++q;
+   const int t = b - ( q > 0 ? mp.end[ q - 1 ] : 0 ), mt = mp.p[ q ].M / 64;
+   gemm_tile<T, BK, 64, 64>( t % mt, t / mt, mp.p[ q ], smem );
 }
 
 // ------------------------------------------------------------------ the diagonal block
@@ -1160,6 +1182,44 @@ template void launch_gemm<double>( hipStream_t, int, int, int, double, const dou
 template void launch_gemm<float>( hipStream_t, int, int, int, float, const float*, long, const float*, long, float, float*,
                                   long, int* );
 
+// The products p[0 .. n) in one k_gemm_multi launch when each would run on 64 x 64 tiles anyway
+// (fewer 128 x 128 tiles than CUs), else one launch_gemm each, in order.
+template <typename T>
+void launch_gemm_multi( hipStream_t st, const GemmProb<T>* p, int n ) {
+   constexpr long kSmall = 256;
+   GemmMulti<T> mp {};
+   bool one = n >= 1 && n <= 4;
+   int tiles = 0;
+   for ( int q = 0; q < n && one; ++q ) {
+      const GemmProb<T>& g = p[ q ];
+      if ( g.M <= 0 || g.N <= 0 ) continue;
+      one = g.M % 64 == 0 && g.N % 64 == 0 && g.K % 64 == 0 && g.C != g.A && g.C != g.B && g.check == nullptr &&
+            (long)( g.M / 128 ) * ( g.N / 128 ) < kSmall;
+      mp.p[ mp.n ] = g;
+      tiles += ( g.M / 64 ) * ( g.N / 64 );
+      mp.end[ mp.n++ ] = tiles;
+   }
+   if ( !one ) {
+      for ( int q = 0; q < n; ++q )
+         launch_gemm<T>( st, p[ q ].M, p[ q ].N, p[ q ].K, p[ q ].alpha, p[ q ].A, p[ q ].lda, p[ q ].B, p[ q ].ldb, p[ q ].beta,
+                         p[ q ].C, p[ q ].ldc, p[ q ].check );
+      return;
+   }
+   if ( tiles == 0 ) return;
+   double flops = 0.0;
+   for ( int q = 0; q < mp.n; ++q ) flops += 2.0 * mp.p[ q ].M * mp.p[ q ].N * mp.p[ q ].K;
+   ProfScope prof( sizeof( T ) == 8 ? kKGemm64 : kKGemm32S, flops, st );
+   if constexpr ( sizeof( T ) == 8 )
+      hipLaunchKernelGGL( ( k_gemm_multi<double, 16> ), dim3( tiles ), dim3( 256 ), 0, st, mp );
+   else if ( g_gemm_bk_f32 == 16 )
+      hipLaunchKernelGGL( ( k_gemm_multi<float, 16> ), dim3( tiles ), dim3( 256 ), 0, st, mp );
+   else if ( g_gemm_bk_f32 == 64 )
+      hipLaunchKernelGGL( ( k_gemm_multi<float, 64> ), dim3( tiles ), dim3( 256 ), 0, st, mp );
+   else
+      hipLaunchKernelGGL( ( k_gemm_multi<float, 32> ), dim3( tiles ), dim3( 256 ), 0, st, mp );
+}
+
+
 // The blocked factorization without exchanges (the fast path).  Block columns are taken in pairs
 // (npad is a multiple of 256): the two 128-column blocks of a pair are factored one after the other
 // (diagonal block, the panel below as A U^-1, and in between the update of the second block
@@ -1276,20 +1336,19 @@ void lu_nopiv( const LuStreams& S, int npad, T* A, long lda, T* inv, int* check 
          const int cr = c2 + nnext / nb, wr = rest - nnext, nn = std::min( 2 * nb, wr );
          const int cr2 = cr + nn / nb, nn2 = std::min( 2 * nb, wr - nn );
          const bool defer = !deferred && wr - nn - nn2 > 0 && rest >= kDeferMinRows;
-         launch_gemm<T>( st2, nnext, wr, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( c2, cr ), lda,
-                         nullptr );
+         // A and N (and when deferring A2 and N2) write disjoint blocks: one launch
+         const GemmProb<T> parts[ 4 ] = {
+            { nnext, wr, 2 * nb, T( -1 ), blk( c2, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( c2, cr ), lda, nullptr, T( 0 ) },
+            { wr, nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( cr, cr ), lda, nullptr, T( 0 ) },
+            { nn, wr - nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr2 ), lda, T( 1 ), blk( cr, cr2 ), lda, nullptr, T( 0 ) },
+            { wr - nn, nn2, 2 * nb, T( -1 ), blk( cr2, k0 ), lda, blk( k0, cr2 ), lda, T( 1 ), blk( cr2, cr2 ), lda, nullptr,
+              T( 0 ) } };
+         launch_gemm_multi<T>( st2, parts, defer ? 4 : 2 );
          BH_HIP( hipEventRecord( S.ev_g, st2 ) );
          pending = true;
-         launch_gemm<T>( st2, wr, nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr ), lda, T( 1 ), blk( cr, cr ), lda,
-                         nullptr );
          BH_HIP( hipEventRecord( S.ev[ 5 ], st2 ) );
          pending_n = true;
-         if ( defer ) {
-            launch_gemm<T>( st2, nn, wr - nn, 2 * nb, T( -1 ), blk( cr, k0 ), lda, blk( k0, cr2 ), lda, T( 1 ), blk( cr, cr2 ),
-                            lda, nullptr );
-            launch_gemm<T>( st2, wr - nn, nn2, 2 * nb, T( -1 ), blk( cr2, k0 ), lda, blk( k0, cr2 ), lda, T( 1 ),
-                            blk( cr2, cr2 ), lda, nullptr );
-         } else if ( wr > nn ) {
+         if ( !defer && wr > nn ) {
             const int kf = deferred ? k0 - 2 : k0;  // first block column whose panels this product applies
             launch_gemm<T>( st2, wr, wr - nn, ( c2 - kf ) * nb, T( -1 ), blk( cr, kf ), lda, blk( kf, cr2 ), lda, T( 1 ),
                             blk( cr, cr2 ), lda, nullptr );
