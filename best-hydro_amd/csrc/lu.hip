@@ -989,8 +989,11 @@ __global__ void k_apply_ipiv( int n, int nrhs, const int* __restrict__ ipiv, T* 
 // ------------------------------------------------------------------ refinement
 
 // Partial products of A x over column chunks: part[c][rhs][row] = sum_{j in chunk c} A(row, j) x(j, rhs).
-// Each thread carries four rows (row0 + 256 m) so every broadcast read of x from LDS feeds four
-// loads of A.
+// Each thread carries kResidRows rows (row0 + 256 m) so every broadcast read of x from LDS feeds
+// that many loads of A (four: one sweep at 5,120 triangles took 0.33 ms; one, two and eight rows
+// per thread made the refinement's two sweeps 0.5, 0.13 and 0.4 ms slower).
+constexpr int kResidRows = 4;
+template <int RM>
 __global__ void __launch_bounds__( 256 ) k_resid_part( int npad, int chunk, int nrhs, const double* __restrict__ A,
                                                       long lda, const double* __restrict__ x, long ldx,
                                                       double* __restrict__ part ) {
@@ -1001,25 +1004,25 @@ __global__ void __launch_bounds__( 256 ) k_resid_part( int npad, int chunk, int 
       xs[ j * 16 + r ] = r < nrhs ? x[ (long)r * ldx + c0 + j ] : 0.0;
    }
    __syncthreads();
-   const int row0 = blockIdx.x * 1024 + threadIdx.x;
-   double acc[ 4 ][ 16 ];
+   const int row0 = blockIdx.x * 256 * RM + threadIdx.x;
+   double acc[ RM ][ 16 ];
 #pragma unroll
-   for ( int m = 0; m < 4; ++m )
+   for ( int m = 0; m < RM; ++m )
 #pragma unroll
       for ( int r = 0; r < 16; ++r ) acc[ m ][ r ] = 0.0;
-   bool ok[ 4 ];
+   bool ok[ RM ];
 #pragma unroll
-   for ( int m = 0; m < 4; ++m ) ok[ m ] = row0 + 256 * m < npad;
+   for ( int m = 0; m < RM; ++m ) ok[ m ] = row0 + 256 * m < npad;
    const double* a = A + (long)c0 * lda + row0;
    // four columns' loads issued before their products (16 loads in flight per lane); the
    // accumulation order over j is unchanged
    constexpr int U = 4;
    for ( int j = 0; j < chunk; j += U ) {
-      double v[ U ][ 4 ];
+      double v[ U ][ RM ];
 #pragma unroll
       for ( int u = 0; u < U; ++u )
 #pragma unroll
-         for ( int m = 0; m < 4; ++m ) v[ u ][ m ] = ok[ m ] ? a[ (long)( j + u ) * lda + 256 * m ] : 0.0;
+         for ( int m = 0; m < RM; ++m ) v[ u ][ m ] = ok[ m ] ? a[ (long)( j + u ) * lda + 256 * m ] : 0.0;
 #pragma unroll
       for ( int u = 0; u < U; ++u ) {
          const double* xr = xs + ( j + u ) * 16;
@@ -1027,13 +1030,13 @@ __global__ void __launch_bounds__( 256 ) k_resid_part( int npad, int chunk, int 
          for ( int r = 0; r < 16; ++r ) {
             const double xv = xr[ r ];
 #pragma unroll
-            for ( int m = 0; m < 4; ++m ) acc[ m ][ r ] = acc[ m ][ r ] + v[ u ][ m ] * xv;
+            for ( int m = 0; m < RM; ++m ) acc[ m ][ r ] = acc[ m ][ r ] + v[ u ][ m ] * xv;
          }
       }
    }
    double* out = part + (long)blockIdx.y * 16 * npad;
 #pragma unroll
-   for ( int m = 0; m < 4; ++m )
+   for ( int m = 0; m < RM; ++m )
 #pragma unroll
       for ( int r = 0; r < 16; ++r )
          if ( ok[ m ] && r < nrhs ) out[ (long)r * npad + row0 + 256 * m ] = acc[ m ][ r ];
@@ -1496,9 +1499,10 @@ void residual( hipStream_t st, int npad, int n3, int nrhs, const double* A, long
                const double* b, long ldb, double* r, long ldr, double* part, double* norms ) {
    const int chunk = resid_chunk( npad );
    const int nchunks = npad / chunk;
-   dim3 grid( ( npad + 1023 ) / 1024, nchunks );
+   constexpr int RM = kResidRows;
+   dim3 grid( ( npad + 256 * RM - 1 ) / ( 256 * RM ), nchunks );
    ProfScope prof( kKResid, 2.0 * npad * npad * nrhs, st );
-   hipLaunchKernelGGL( k_resid_part, grid, dim3( 256 ), chunk * 16 * sizeof( double ), st, npad, chunk, nrhs, A, lda, x,
+   hipLaunchKernelGGL( k_resid_part<RM>, grid, dim3( 256 ), chunk * 16 * sizeof( double ), st, npad, chunk, nrhs, A, lda, x,
                        ldx, part );
    hipLaunchKernelGGL( k_resid_final, dim3( ( npad + 255 ) / 256, nrhs ), dim3( 256 ), 0, st, npad, n3, nchunks, nrhs, part,
                        b, ldb, r, ldr, norms );
