@@ -366,7 +366,7 @@ def main():
     traffic, traffic_src = traffic_from_profile(dom)
     kernels = {"gemm32_128": "k_gemm<float, 16, 128, 128> (FP32 trailing updates, 128 x 128 tiles)",
                "gemm32": "k_gemm<float, 32, 64, 128> and <float, 32, 128, 64> (FP32 panels and U row panels)",
-               "gemm32_64": "k_gemm<float, 32, 64, 64> (FP32 narrow update parts, 64 x 64 tiles)",
+               "gemm32_64": "k_gemm<float, 32, 64, 64> and k_gemm_multi<float, 32> (FP32 narrow update parts, 64 x 64 tiles)",
                "pairs": "k_pairs<true> (assembly)", "diag": "k_lu_diag<float>"}
     roofline = {"bound": "mfma" if dom.startswith("gemm") else "valu", "kernel_class": dom,
                 "kernel": kernels.get(dom, dom),
