@@ -901,6 +901,61 @@ __global__ void __launch_bounds__( 256 ) k_skinny_part( int M, int K, int kper, 
    }
 }
 
+// Y = beta Y + alpha A X in one launch, no partial sums in memory: A column-major M x K, X and Y
+// row-interleaved; M a multiple of 16, K of 32.  Workgroup: 16 rows and all of K; its eight waves
+// split K, each one 16 x 16 x 4 MFMA tile with every load of its K range in flight at once (for
+// K = 1024 and FP32: 32 per lane), and wave 0 adds the eight waves' tiles from LDS in a fixed
+// order.  For the diagonal super-block steps (M = K = 1024) this is 64 workgroups and one launch
+// where the split-K form needs a product and a reduce launch.  A workgroup's rows are 64 bytes of
+// each column, half a 128-byte line: when the row blocks are a multiple of 16, blocks 2j and
+// 2j + 1 go to workgroups 8 apart, which the dispatcher places on the same XCD (consecutive
+// workgroups go to the 8 XCDs in turn), so both halves of a line meet in one L2.
+template <typename T>
+__global__ void __launch_bounds__( 512 ) k_skinny_rows( int K, T alpha, const T* __restrict__ A, long lda,
+                                                       const T* __restrict__ X, T beta, T* __restrict__ Y ) {
+   using v4 = typename Mfma<T>::v4;
+   constexpr int W = 8, PF = sizeof( T ) == 4 ? 32 : 16;
+   __shared__ T red[ W - 1 ][ 256 ];
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   int rb = blockIdx.x;
+   if ( gridDim.x % 16 == 0 ) rb = ( rb & ~15 ) + 2 * ( rb & 7 ) + ( ( rb >> 3 ) & 1 );
+   const int row0 = rb * 16;
+   const int kw = K / W;
+   const int li = lane & 15, lk = lane >> 4;
+   const T* a = A + (long)( w * kw + lk ) * lda + row0 + li;
+   const T* x = X + (long)( w * kw + lk ) * 16 + li;
+   v4 acc = v4{ 0, 0, 0, 0 };
+   for ( int k = 0; k < kw; k += 4 * PF ) {
+      T fa[ PF ], fb[ PF ];
+#pragma unroll
+      for ( int u = 0; u < PF; ++u ) {
+         const bool in = k + 4 * u < kw;
+         fa[ u ] = in ? a[ (long)( k + 4 * u ) * lda ] : T( 0 );
+         fb[ u ] = in ? x[ (long)( k + 4 * u ) * 16 ] : T( 0 );
+      }
+#pragma unroll
+      for ( int u = 0; u < PF; ++u ) acc = Mfma<T>::run( fa[ u ], fb[ u ], acc );
+   }
+   // accumulator element r of this lane: row (FP32) 4 (lane/16) + r or (FP64) lane/16 + 4 r,
+   // right-hand side lane % 16
+   auto rowof = [&]( int r ) { return sizeof( T ) == 8 ? lk + 4 * r : 4 * lk + r; };
+   if ( w > 0 ) {
+#pragma unroll
+      for ( int r = 0; r < 4; ++r ) red[ w - 1 ][ rowof( r ) * 16 + li ] = acc[ r ];
+   }
+   __syncthreads();
+   if ( w == 0 ) {
+#pragma unroll
+      for ( int r = 0; r < 4; ++r ) {
+         const int e = rowof( r ) * 16 + li;
+         const T s = ( ( acc[ r ] + red[ 0 ][ e ] ) + ( red[ 1 ][ e ] + red[ 2 ][ e ] ) ) +
+                     ( ( red[ 3 ][ e ] + red[ 4 ][ e ] ) + ( red[ 5 ][ e ] + red[ 6 ][ e ] ) );
+         T* y = Y + (long)row0 * 16 + e;
+         *y = beta == T( 0 ) ? alpha * s : beta * *y + alpha * s;
+      }
+   }
+}
+
 // Y = beta Y + alpha sum_kc P[kc] (row-interleaved, M x 16), the slices added in order.
 template <typename T>
 __global__ void __launch_bounds__( 256 ) k_skinny_reduce( long n16, int nkc, T alpha, const T* __restrict__ P, T beta,
@@ -1411,6 +1466,15 @@ void launch_skinny( hipStream_t st, int M, int K, T alpha, const T* A, long lda,
                        n16, nkc, alpha, P, beta, Y );
 }
 
+// The same product as one k_skinny_rows launch (M a multiple of 16, K of 32).
+template <typename T>
+void launch_skinny_rows( hipStream_t st, int M, int K, T alpha, const T* A, long lda, const T* X, T beta, T* Y ) {
+   if ( M <= 0 ) return;
+   if ( M % 16 != 0 || K % 32 != 0 ) throw std::runtime_error( "launch_skinny_rows: M must be a multiple of 16, K of 32" );
+   ProfScope prof( kKTrsm, 2.0 * M * K * 11, st );
+   hipLaunchKernelGGL( k_skinny_rows<T>, dim3( M / 16 ), dim3( 512 ), 0, st, K, alpha, A, lda, X, beta, Y );
+}
+
 // Inverses of the S x S diagonal super-blocks of L (unit lower) and U, from the 128 x 128 block
 // inverses by recursive doubling: with A, B the inverses of two adjacent s x s diagonal blocks and
 // C the block between them, inv([A 0; C B]) has -B^-1 C A^-1 below (and inv of upper: -A^-1 C B^-1
@@ -1447,8 +1511,9 @@ template void build_super_inverses<float>( hipStream_t, int, int, const float*, 
 
 // Solves L U x = P b with the factors and the super-block inverses.  b, y, x are row-interleaved
 // npad x 16 blocks: b is consumed (row-exchanged when pivoted, then overwritten by the forward
-// updates), y is work space, the solution goes to x.  Two launches per super-block and triangle:
-// the inverse times the block, then the update of the rest of the right-hand side.
+// updates), y is work space, the solution goes to x.  Two products per super-block and triangle:
+// the inverse times the block (one k_skinny_rows launch), then the update of the rest of the
+// right-hand side (split-K: a product and a reduce launch).
 // piv: 0 no exchanges; 1 P b first (A = P^T L U); -1 the factors are those of A^T = U^T L^T P
 // transposed in place (the packed factors and the super-block inverses transposed, XL and XU
 // exchanged), so P^T is applied to the solution last.
@@ -1459,13 +1524,13 @@ void lu_solve( hipStream_t st, int npad, int n3, int S, const T* LU, long lda, c
    if ( piv == 1 ) hipLaunchKernelGGL( k_apply_ipiv_il<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, ipiv, b, 0 );
    for ( int q = 0; q < nsb; ++q ) {
       const long q0 = (long)q * S;
-      launch_skinny<T>( st, S, S, T( 1 ), XL + q * (long)S * S, S, b + q0 * 16, T( 0 ), y + q0 * 16, P );
+      launch_skinny_rows<T>( st, S, S, T( 1 ), XL + q * (long)S * S, S, b + q0 * 16, T( 0 ), y + q0 * 16 );
       launch_skinny<T>( st, npad - (int)q0 - S, S, T( -1 ), LU + q0 * lda + q0 + S, lda, y + q0 * 16, T( 1 ),
                         b + ( q0 + S ) * 16, P );
    }
    for ( int q = nsb - 1; q >= 0; --q ) {
       const long q0 = (long)q * S;
-      launch_skinny<T>( st, S, S, T( 1 ), XU + q * (long)S * S, S, y + q0 * 16, T( 0 ), x + q0 * 16, P );
+      launch_skinny_rows<T>( st, S, S, T( 1 ), XU + q * (long)S * S, S, y + q0 * 16, T( 0 ), x + q0 * 16 );
       launch_skinny<T>( st, (int)q0, S, T( -1 ), LU + q0 * lda, lda, x + q0 * 16, T( 1 ), y, P );
    }
    if ( piv == -1 ) hipLaunchKernelGGL( k_apply_ipiv_il<T>, dim3( 1 ), dim3( 64 ), 0, st, n3, ipiv, x, 1 );
