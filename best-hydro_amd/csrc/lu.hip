@@ -902,19 +902,19 @@ __global__ void __launch_bounds__( 256 ) k_skinny_part( int M, int K, int kper, 
 }
 
 // Y = beta Y + alpha A X in one launch, no partial sums in memory: A column-major M x K, X and Y
-// row-interleaved; M a multiple of 16, K of 32.  Workgroup: 16 rows and all of K; its eight waves
-// split K, each one 16 x 16 x 4 MFMA tile with every load of its K range in flight at once (for
-// K = 1024 and FP32: 32 per lane), and wave 0 adds the eight waves' tiles from LDS in a fixed
-// order.  For the diagonal super-block steps (M = K = 1024) this is 64 workgroups and one launch
+// row-interleaved; M a multiple of 16, K of 64.  Workgroup: 16 rows and all of K; its sixteen
+// waves split K, each one 16 x 16 x 4 MFMA tile with every load of its K range in flight at once
+// (for K = 1024: 16 per lane), and wave 0 adds the sixteen waves' tiles from LDS in a fixed order
+// (eight waves of 32 loads each: the solve phase 0.03 ms longer).  For the diagonal super-block steps (M = K = 1024) this is 64 workgroups and one launch
 // where the split-K form needs a product and a reduce launch.  A workgroup's rows are 64 bytes of
 // each column, half a 128-byte line: when the row blocks are a multiple of 16, blocks 2j and
 // 2j + 1 go to workgroups 8 apart, which the dispatcher places on the same XCD (consecutive
 // workgroups go to the 8 XCDs in turn), so both halves of a line meet in one L2.
 template <typename T>
-__global__ void __launch_bounds__( 512 ) k_skinny_rows( int K, T alpha, const T* __restrict__ A, long lda,
-                                                       const T* __restrict__ X, T beta, T* __restrict__ Y ) {
+__global__ void __launch_bounds__( 1024 ) k_skinny_rows( int K, T alpha, const T* __restrict__ A, long lda,
+                                                        const T* __restrict__ X, T beta, T* __restrict__ Y ) {
    using v4 = typename Mfma<T>::v4;
-   constexpr int W = 8, PF = sizeof( T ) == 4 ? 32 : 16;
+   constexpr int W = 16, PF = 16;
    __shared__ T red[ W - 1 ][ 256 ];
    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
    int rb = blockIdx.x;
@@ -948,8 +948,10 @@ __global__ void __launch_bounds__( 512 ) k_skinny_rows( int K, T alpha, const T*
 #pragma unroll
       for ( int r = 0; r < 4; ++r ) {
          const int e = rowof( r ) * 16 + li;
-         const T s = ( ( acc[ r ] + red[ 0 ][ e ] ) + ( red[ 1 ][ e ] + red[ 2 ][ e ] ) ) +
-                     ( ( red[ 3 ][ e ] + red[ 4 ][ e ] ) + ( red[ 5 ][ e ] + red[ 6 ][ e ] ) );
+         const T s = ( ( ( acc[ r ] + red[ 0 ][ e ] ) + ( red[ 1 ][ e ] + red[ 2 ][ e ] ) ) +
+                       ( ( red[ 3 ][ e ] + red[ 4 ][ e ] ) + ( red[ 5 ][ e ] + red[ 6 ][ e ] ) ) ) +
+                     ( ( ( red[ 7 ][ e ] + red[ 8 ][ e ] ) + ( red[ 9 ][ e ] + red[ 10 ][ e ] ) ) +
+                       ( ( red[ 11 ][ e ] + red[ 12 ][ e ] ) + ( red[ 13 ][ e ] + red[ 14 ][ e ] ) ) );
          T* y = Y + (long)row0 * 16 + e;
          *y = beta == T( 0 ) ? alpha * s : beta * *y + alpha * s;
       }
@@ -1466,13 +1468,13 @@ void launch_skinny( hipStream_t st, int M, int K, T alpha, const T* A, long lda,
                        n16, nkc, alpha, P, beta, Y );
 }
 
-// The same product as one k_skinny_rows launch (M a multiple of 16, K of 32).
+// The same product as one k_skinny_rows launch (M a multiple of 16, K of 64).
 template <typename T>
 void launch_skinny_rows( hipStream_t st, int M, int K, T alpha, const T* A, long lda, const T* X, T beta, T* Y ) {
    if ( M <= 0 ) return;
-   if ( M % 16 != 0 || K % 32 != 0 ) throw std::runtime_error( "launch_skinny_rows: M must be a multiple of 16, K of 32" );
+   if ( M % 16 != 0 || K % 64 != 0 ) throw std::runtime_error( "launch_skinny_rows: M must be a multiple of 16, K of 64" );
    ProfScope prof( kKTrsm, 2.0 * M * K * 11, st );
-   hipLaunchKernelGGL( k_skinny_rows<T>, dim3( M / 16 ), dim3( 512 ), 0, st, K, alpha, A, lda, X, beta, Y );
+   hipLaunchKernelGGL( k_skinny_rows<T>, dim3( M / 16 ), dim3( 1024 ), 0, st, K, alpha, A, lda, X, beta, Y );
 }
 
 // Inverses of the S x S diagonal super-blocks of L (unit lower) and U, from the 128 x 128 block
